@@ -1,0 +1,749 @@
+// als_api.cpp -- host driver of the MI355X ALS update.
+//
+// Reference: the half-sweep driver doALSWithCSR (als/src/cuda/als.cu:1000-
+// 1115) and its batching loop (:1061-1100).  The reference allocates,
+// creates BLAS handles and synchronises around every step; here a half-
+// sweep is two kernel launches on one stream (fused Gram+solve over
+// wavefront tasks, then the reduction of split rows), with the ratings and
+// both factor matrices resident on the device between sweeps.
+//
+// Task planning (host, once per rating matrix): every row with at most
+// `split` ratings becomes one task; longer rows are cut into `split`-sized
+// chunks whose partial Grams land in scratch slots and are summed in chunk
+// order by the finish kernel.  Tasks are ordered by decreasing length so
+// the longest start first (longest-processing-time-first scheduling).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/als.h"
+#include "../../include/utilities.h"
+#include "als_kernels.h"
+#include "cumf_internal.h"
+
+namespace {
+
+constexpr int kMaxF = 140;
+constexpr int kDefaultSplit = 2048;
+
+int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }
+
+// Device allocation owned by the host driver.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t n) {
+    reset();
+    if (n == 0) return CUMF_OK;
+    if (hipMalloc(&p, n) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      cumf::set_last_error("hipMalloc of " + std::to_string(n) + " bytes failed");
+      return CUMF_ERR_NOMEM;
+    }
+    bytes = n;
+    return CUMF_OK;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+#define CUMF_TRY(expr)          \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != CUMF_OK) return rc_; \
+  } while (0)
+
+// Wavefront task list of one rating matrix side.
+struct Plan {
+  int64_t rows = 0, nnz = 0, ntasks = 0, nheavy = 0, nslots = 0;
+  DevBuf tasks, heavy, partial;
+
+  int build(const int32_t* row_ptr, int64_t m, int split, int f) {
+    std::vector<cumf::Task> t;
+    std::vector<cumf::HeavyRow> hv;
+    t.reserve((size_t)m);
+    int64_t slot = 0;
+    for (int64_t u = 0; u < m; ++u) {
+      const int64_t n = (int64_t)row_ptr[u + 1] - row_ptr[u];
+      if (n < 0) {
+        cumf::set_last_error("row pointers decrease at row " + std::to_string(u));
+        return CUMF_ERR_INVALID;
+      }
+      if (n <= split) {
+        t.push_back({(int32_t)u, 0, (int32_t)n, -1});
+      } else {
+        const int64_t nc = (n + split - 1) / split;
+        hv.push_back({(int32_t)u, (int32_t)slot, (int32_t)nc, 0});
+        for (int64_t c = 0; c < nc; ++c)
+          t.push_back({(int32_t)u, (int32_t)(c * split), (int32_t)std::min<int64_t>(split, n - c * split),
+                       (int32_t)(slot + c)});
+        slot += nc;
+      }
+    }
+    std::stable_sort(t.begin(), t.end(),
+                     [](const cumf::Task& a, const cumf::Task& b) { return a.count > b.count; });
+    rows = m;
+    nnz = row_ptr[m];
+    ntasks = (int64_t)t.size();
+    nheavy = (int64_t)hv.size();
+    nslots = slot;
+    CUMF_TRY(tasks.alloc(sizeof(cumf::Task) * t.size()));
+    CUMF_TRY(heavy.alloc(sizeof(cumf::HeavyRow) * hv.size()));
+    CUMF_TRY(partial.alloc(sizeof(float) * cumf::partial_floats_per_slot(f) * (size_t)nslots));
+    if (!t.empty()) CUMF_HIP_TRY(hipMemcpy(tasks.p, t.data(), tasks.bytes, hipMemcpyHostToDevice));
+    if (!hv.empty()) CUMF_HIP_TRY(hipMemcpy(heavy.p, hv.data(), heavy.bytes, hipMemcpyHostToDevice));
+    return CUMF_OK;
+  }
+};
+
+int check_f(int f) {
+  if (f < 1 || f > kMaxF) {
+    cumf::set_last_error("f must be in [1, " + std::to_string(kMaxF) + "], got " + std::to_string(f));
+    return CUMF_ERR_INVALID;
+  }
+  return CUMF_OK;
+}
+
+// Kernel-time bookkeeping with HIP events.
+struct Timer {
+  enum Kind { TASK = 0, FINISH, RMSE, SWEEP, NKINDS };
+  struct Pending { hipEvent_t a, b; int kind; };
+  std::vector<hipEvent_t> pool;
+  std::vector<Pending> pending;
+  double ms[NKINDS] = {0, 0, 0, 0};
+  int64_t count[NKINDS] = {0, 0, 0, 0};
+
+  ~Timer() {
+    for (auto& p : pending) { pool.push_back(p.a); pool.push_back(p.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+    return e;
+  }
+  // Must be called after the work the events bracket has completed.
+  void resolve() {
+    for (auto& p : pending) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) { ms[p.kind] += t; count[p.kind]++; }
+      else (void)hipGetLastError();
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+};
+
+}  // namespace
+
+struct cumf_handle {
+  int device = 0;
+  int f = 0;
+  int64_t ldf = 0;
+  float lambda = 0.f;
+  cumf_options opt{};
+  hipStream_t stream = nullptr;
+  int64_t m = 0, n = 0, nnz = 0, ntest = 0;
+  DevBuf rp, ci, rv, cp, ri, cv;  // CSR and CSC
+  DevBuf X, T;                    // factors, row stride ldf
+  DevBuf tu, tv, tr;              // held-out COO
+  DevBuf rmse_partial, rmse_out, fail;
+  Plan plan_x, plan_t;
+  Timer timer;
+  int64_t gram_flops = 0;
+  bool have_ratings = false;
+};
+
+namespace {
+
+constexpr int kRmseBlocks = 1024;
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) { (void)hipGetLastError(); prev = -1; }
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+template <class F>
+int timed(cumf_handle* h, int kind, F&& launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (h->opt.timing) {
+    a = h->timer.get();
+    b = h->timer.get();
+    if (a) (void)hipEventRecord(a, h->stream);
+  }
+  hipError_t e = launch();
+  if (e != hipSuccess) {
+    cumf::set_last_error(std::string("kernel launch failed: ") + hipGetErrorString(e));
+    if (a) h->timer.pool.push_back(a);
+    if (b) h->timer.pool.push_back(b);
+    return CUMF_ERR_HIP;
+  }
+  if (a && b) {
+    (void)hipEventRecord(b, h->stream);
+    h->timer.pending.push_back({a, b, kind});
+  }
+  return CUMF_OK;
+}
+
+int run_half_sweep(cumf_handle* h, const Plan& p, const DevBuf& rowp, const DevBuf& col,
+                   const DevBuf& val, const float* theta, float* x) {
+  cumf::SweepArgs a{};
+  a.row_ptr = rowp.as<int32_t>();
+  a.col = col.as<int32_t>();
+  a.val = val.as<float>();
+  a.theta = theta;
+  a.x = x;
+  a.ldf = h->ldf;
+  a.tasks = p.tasks.as<cumf::Task>();
+  a.ntasks = p.ntasks;
+  a.heavy = p.heavy.as<cumf::HeavyRow>();
+  a.nheavy = p.nheavy;
+  a.partial = p.partial.as<float>();
+  a.fail_counter = h->fail.as<int>();
+  a.f = h->f;
+  a.lambda = h->lambda;
+  hipEvent_t s0 = nullptr, s1 = nullptr;
+  if (h->opt.timing) {
+    s0 = h->timer.get();
+    s1 = h->timer.get();
+    if (s0) (void)hipEventRecord(s0, h->stream);
+  }
+  CUMF_TRY(timed(h, Timer::TASK, [&] { return cumf::launch_task_kernel(a, h->stream); }));
+  if (p.nheavy > 0)
+    CUMF_TRY(timed(h, Timer::FINISH, [&] { return cumf::launch_finish_kernel(a, h->stream); }));
+  if (s0 && s1) {
+    (void)hipEventRecord(s1, h->stream);
+    h->timer.pending.push_back({s0, s1, Timer::SWEEP});
+  }
+  h->gram_flops += 2 * p.nnz * (int64_t)h->f * (h->f + 1);
+  return CUMF_OK;
+}
+
+int upload_factors(cumf_handle* h, DevBuf& dst, const float* src, int64_t rows) {
+  CUMF_HIP_TRY(hipMemsetAsync(dst.p, 0, dst.bytes, h->stream));
+  CUMF_HIP_TRY(hipMemcpy2DAsync(dst.p, sizeof(float) * h->ldf, src, sizeof(float) * h->f,
+                                sizeof(float) * h->f, rows, hipMemcpyHostToDevice, h->stream));
+  return CUMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cumf_default_options(cumf_options* opt) {
+  if (!opt) return CUMF_ERR_INVALID;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->version = CUMF_OPTIONS_VERSION;
+  opt->precision = CUMF_PREC_FP32;
+  opt->split = kDefaultSplit;
+  opt->timing = 0;
+  return CUMF_OK;
+}
+
+int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_handle** out) {
+  cumf::clear_last_error();
+  if (!out) return CUMF_ERR_INVALID;
+  *out = nullptr;
+  CUMF_TRY(check_f(f));
+  if (!(lambda >= 0.f)) {
+    cumf::set_last_error("lambda must be >= 0");
+    return CUMF_ERR_INVALID;
+  }
+  cumf_options o;
+  cumf_default_options(&o);
+  if (opt) {
+    if (opt->version != CUMF_OPTIONS_VERSION) {
+      cumf::set_last_error("cumf_options.version mismatch");
+      return CUMF_ERR_INVALID;
+    }
+    o = *opt;
+    if (o.split <= 0) o.split = kDefaultSplit;
+  }
+  if (o.precision != CUMF_PREC_FP32) {
+    cumf::set_last_error("precision mode not built");
+    return CUMF_ERR_UNSUPPORTED;
+  }
+  if (device < 0) device = get_gpu();
+  int count = 0;
+  if (device < 0 || hipGetDeviceCount(&count) != hipSuccess || device >= count) {
+    (void)hipGetLastError();
+    cumf::set_last_error("no usable GPU device");
+    return CUMF_ERR_NODEVICE;
+  }
+  DeviceGuard dg(device);
+  auto* h = new cumf_handle();
+  h->device = device;
+  h->f = f;
+  h->ldf = round_up4(f);
+  h->lambda = lambda;
+  h->opt = o;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    delete h;
+    cumf::set_last_error("hipStreamCreate failed");
+    return CUMF_ERR_HIP;
+  }
+  int rc = h->fail.alloc(sizeof(int));
+  if (rc == CUMF_OK) rc = h->rmse_partial.alloc(sizeof(double) * kRmseBlocks);
+  if (rc == CUMF_OK) rc = h->rmse_out.alloc(sizeof(double));
+  if (rc == CUMF_OK && hipMemset(h->fail.p, 0, sizeof(int)) != hipSuccess) rc = CUMF_ERR_HIP;
+  if (rc != CUMF_OK) {
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return CUMF_OK;
+}
+
+int cumf_destroy(cumf_handle* h) {
+  if (!h) return CUMF_OK;
+  {
+    DeviceGuard dg(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    h->timer.resolve();
+    (void)hipStreamDestroy(h->stream);
+    h->stream = nullptr;
+    delete h;  // DevBuf destructors free on the handle's device
+  }
+  return CUMF_OK;
+}
+
+int cumf_set_ratings(cumf_handle* h, int64_t m, int64_t n, int64_t nnz, const int32_t* csr_row_ptr,
+                     const int32_t* csr_col, const float* csr_val, const int32_t* csc_col_ptr,
+                     const int32_t* csc_row, const float* csc_val) {
+  cumf::clear_last_error();
+  if (!h || m <= 0 || n <= 0 || nnz < 0 || nnz > INT32_MAX || m > INT32_MAX || n > INT32_MAX ||
+      !csr_row_ptr || (nnz > 0 && (!csr_col || !csr_val))) {
+    cumf::set_last_error("cumf_set_ratings: bad arguments");
+    return CUMF_ERR_INVALID;
+  }
+  if (csr_row_ptr[0] != 0 || csr_row_ptr[m] != nnz) {
+    cumf::set_last_error("cumf_set_ratings: row pointers must start at 0 and end at nnz");
+    return CUMF_ERR_INVALID;
+  }
+  for (int64_t k = 0; k < nnz; ++k)
+    if (csr_col[k] < 0 || csr_col[k] >= n) {
+      cumf::set_last_error("cumf_set_ratings: column index out of range at " + std::to_string(k));
+      return CUMF_ERR_INVALID;
+    }
+  // CSC: given, or built here.
+  std::vector<int32_t> cp_b, ri_b;
+  std::vector<float> cv_b;
+  if (!csc_col_ptr) {
+    cp_b.assign(n + 1, 0);
+    for (int64_t k = 0; k < nnz; ++k) cp_b[csr_col[k] + 1]++;
+    for (int64_t j = 0; j < n; ++j) cp_b[j + 1] += cp_b[j];
+    std::vector<int64_t> pos(cp_b.begin(), cp_b.end() - 1);
+    ri_b.resize(nnz);
+    cv_b.resize(nnz);
+    for (int64_t u = 0; u < m; ++u)
+      for (int64_t k = csr_row_ptr[u]; k < csr_row_ptr[u + 1]; ++k) {
+        int64_t d = pos[csr_col[k]]++;
+        ri_b[d] = (int32_t)u;
+        cv_b[d] = csr_val[k];
+      }
+    csc_col_ptr = cp_b.data();
+    csc_row = ri_b.data();
+    csc_val = cv_b.data();
+  } else {
+    if (nnz > 0 && (!csc_row || !csc_val)) {
+      cumf::set_last_error("cumf_set_ratings: CSC arrays missing");
+      return CUMF_ERR_INVALID;
+    }
+    if (csc_col_ptr[0] != 0 || csc_col_ptr[n] != nnz) {
+      cumf::set_last_error("cumf_set_ratings: CSC column pointers must start at 0 and end at nnz");
+      return CUMF_ERR_INVALID;
+    }
+    for (int64_t k = 0; k < nnz; ++k)
+      if (csc_row[k] < 0 || csc_row[k] >= m) {
+        cumf::set_last_error("cumf_set_ratings: CSC row index out of range at " + std::to_string(k));
+        return CUMF_ERR_INVALID;
+      }
+  }
+  DeviceGuard dg(h->device);
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  h->have_ratings = false;
+  CUMF_TRY(h->rp.alloc(sizeof(int32_t) * (m + 1)));
+  CUMF_TRY(h->ci.alloc(sizeof(int32_t) * nnz));
+  CUMF_TRY(h->rv.alloc(sizeof(float) * nnz));
+  CUMF_TRY(h->cp.alloc(sizeof(int32_t) * (n + 1)));
+  CUMF_TRY(h->ri.alloc(sizeof(int32_t) * nnz));
+  CUMF_TRY(h->cv.alloc(sizeof(float) * nnz));
+  CUMF_TRY(h->X.alloc(sizeof(float) * m * h->ldf));
+  CUMF_TRY(h->T.alloc(sizeof(float) * n * h->ldf));
+  CUMF_HIP_TRY(hipMemcpy(h->rp.p, csr_row_ptr, h->rp.bytes, hipMemcpyHostToDevice));
+  if (nnz > 0) {
+    CUMF_HIP_TRY(hipMemcpy(h->ci.p, csr_col, h->ci.bytes, hipMemcpyHostToDevice));
+    CUMF_HIP_TRY(hipMemcpy(h->rv.p, csr_val, h->rv.bytes, hipMemcpyHostToDevice));
+    CUMF_HIP_TRY(hipMemcpy(h->ri.p, csc_row, h->ri.bytes, hipMemcpyHostToDevice));
+    CUMF_HIP_TRY(hipMemcpy(h->cv.p, csc_val, h->cv.bytes, hipMemcpyHostToDevice));
+  }
+  CUMF_HIP_TRY(hipMemcpy(h->cp.p, csc_col_ptr, h->cp.bytes, hipMemcpyHostToDevice));
+  CUMF_HIP_TRY(hipMemset(h->X.p, 0, h->X.bytes));
+  CUMF_HIP_TRY(hipMemset(h->T.p, 0, h->T.bytes));
+  CUMF_TRY(h->plan_x.build(csr_row_ptr, m, h->opt.split, h->f));
+  CUMF_TRY(h->plan_t.build(csc_col_ptr, n, h->opt.split, h->f));
+  h->m = m;
+  h->n = n;
+  h->nnz = nnz;
+  h->have_ratings = true;
+  return CUMF_OK;
+}
+
+int cumf_set_heldout(cumf_handle* h, int64_t count, const int32_t* u, const int32_t* v,
+                     const float* r) {
+  cumf::clear_last_error();
+  if (!h || !h->have_ratings || count < 0 || (count > 0 && (!u || !v || !r))) {
+    cumf::set_last_error("cumf_set_heldout: bad arguments or no ratings yet");
+    return CUMF_ERR_INVALID;
+  }
+  for (int64_t k = 0; k < count; ++k)
+    if (u[k] < 0 || u[k] >= h->m || v[k] < 0 || v[k] >= h->n) {
+      cumf::set_last_error("cumf_set_heldout: index out of range at " + std::to_string(k));
+      return CUMF_ERR_INVALID;
+    }
+  DeviceGuard dg(h->device);
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  CUMF_TRY(h->tu.alloc(sizeof(int32_t) * count));
+  CUMF_TRY(h->tv.alloc(sizeof(int32_t) * count));
+  CUMF_TRY(h->tr.alloc(sizeof(float) * count));
+  if (count > 0) {
+    CUMF_HIP_TRY(hipMemcpy(h->tu.p, u, h->tu.bytes, hipMemcpyHostToDevice));
+    CUMF_HIP_TRY(hipMemcpy(h->tv.p, v, h->tv.bytes, hipMemcpyHostToDevice));
+    CUMF_HIP_TRY(hipMemcpy(h->tr.p, r, h->tr.bytes, hipMemcpyHostToDevice));
+  }
+  h->ntest = count;
+  return CUMF_OK;
+}
+
+int cumf_set_factors(cumf_handle* h, const float* X, const float* theta) {
+  cumf::clear_last_error();
+  if (!h || !h->have_ratings) {
+    cumf::set_last_error("cumf_set_factors: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  DeviceGuard dg(h->device);
+  if (X) CUMF_TRY(upload_factors(h, h->X, X, h->m));
+  if (theta) CUMF_TRY(upload_factors(h, h->T, theta, h->n));
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  return CUMF_OK;
+}
+
+int cumf_get_factors(cumf_handle* h, float* X, float* theta) {
+  cumf::clear_last_error();
+  if (!h || !h->have_ratings) {
+    cumf::set_last_error("cumf_get_factors: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  DeviceGuard dg(h->device);
+  if (X)
+    CUMF_HIP_TRY(hipMemcpy2DAsync(X, sizeof(float) * h->f, h->X.p, sizeof(float) * h->ldf,
+                                  sizeof(float) * h->f, h->m, hipMemcpyDeviceToHost, h->stream));
+  if (theta)
+    CUMF_HIP_TRY(hipMemcpy2DAsync(theta, sizeof(float) * h->f, h->T.p, sizeof(float) * h->ldf,
+                                  sizeof(float) * h->f, h->n, hipMemcpyDeviceToHost, h->stream));
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  return CUMF_OK;
+}
+
+int cumf_update_x(cumf_handle* h) {
+  if (!h || !h->have_ratings) {
+    cumf::set_last_error("cumf_update_x: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  DeviceGuard dg(h->device);
+  return run_half_sweep(h, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>());
+}
+
+int cumf_update_theta(cumf_handle* h) {
+  if (!h || !h->have_ratings) {
+    cumf::set_last_error("cumf_update_theta: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  DeviceGuard dg(h->device);
+  return run_half_sweep(h, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>());
+}
+
+int cumf_iterate(cumf_handle* h, int iters) {
+  for (int it = 0; it < iters; ++it) {
+    CUMF_TRY(cumf_update_x(h));
+    CUMF_TRY(cumf_update_theta(h));
+  }
+  return CUMF_OK;
+}
+
+int cumf_heldout_rmse(cumf_handle* h, double* rmse) {
+  cumf::clear_last_error();
+  if (!h || !rmse || !h->have_ratings) {
+    cumf::set_last_error("cumf_heldout_rmse: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  if (h->ntest == 0) {
+    *rmse = 0.0;
+    return CUMF_OK;
+  }
+  DeviceGuard dg(h->device);
+  CUMF_TRY(timed(h, Timer::RMSE, [&] {
+    return cumf::launch_rmse(h->tu.as<int32_t>(), h->tv.as<int32_t>(), h->tr.as<float>(), h->ntest,
+                             h->X.as<float>(), h->T.as<float>(), h->ldf, h->f,
+                             h->rmse_partial.as<double>(), kRmseBlocks, h->rmse_out.as<double>(),
+                             h->stream);
+  }));
+  double se = 0.0;
+  CUMF_HIP_TRY(hipMemcpyAsync(&se, h->rmse_out.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  *rmse = std::sqrt(se / (double)h->ntest);
+  return CUMF_OK;
+}
+
+int cumf_synchronize(cumf_handle* h) {
+  if (!h) return CUMF_ERR_INVALID;
+  DeviceGuard dg(h->device);
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  h->timer.resolve();
+  return CUMF_OK;
+}
+
+int cumf_get_stats(cumf_handle* h, cumf_stats* s) {
+  if (!h || !s) return CUMF_ERR_INVALID;
+  CUMF_TRY(cumf_synchronize(h));
+  DeviceGuard dg(h->device);
+  int fails = 0;
+  CUMF_HIP_TRY(hipMemcpy(&fails, h->fail.p, sizeof(int), hipMemcpyDeviceToHost));
+  std::memset(s, 0, sizeof(*s));
+  s->task_ms = h->timer.ms[Timer::TASK];
+  s->task_launches = h->timer.count[Timer::TASK];
+  s->finish_ms = h->timer.ms[Timer::FINISH];
+  s->finish_launches = h->timer.count[Timer::FINISH];
+  s->rmse_ms = h->timer.ms[Timer::RMSE];
+  s->rmse_launches = h->timer.count[Timer::RMSE];
+  s->sweep_ms = h->timer.ms[Timer::SWEEP];
+  s->sweeps = h->timer.count[Timer::SWEEP];
+  s->failed_rows = fails;
+  s->gram_flops = h->gram_flops;
+  return CUMF_OK;
+}
+
+int cumf_reset_stats(cumf_handle* h) {
+  if (!h) return CUMF_ERR_INVALID;
+  CUMF_TRY(cumf_synchronize(h));
+  DeviceGuard dg(h->device);
+  for (int k = 0; k < Timer::NKINDS; ++k) {
+    h->timer.ms[k] = 0.0;
+    h->timer.count[k] = 0;
+  }
+  h->gram_flops = 0;
+  CUMF_HIP_TRY(hipMemset(h->fail.p, 0, sizeof(int)));
+  return CUMF_OK;
+}
+
+hipStream_t cumf_get_stream(cumf_handle* h) { return h ? h->stream : nullptr; }
+
+int cumf_debug_gram_dim(int f) { return (f >= 1 && f <= kMaxF) ? cumf::dump_dim(f) : -1; }
+
+int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t nrows, float* out) {
+  cumf::clear_last_error();
+  if (!h || !h->have_ratings || (side != 0 && side != 1) || nrows < 0 || (nrows > 0 && (!rows || !out))) {
+    cumf::set_last_error("cumf_debug_row_gram: bad arguments");
+    return CUMF_ERR_INVALID;
+  }
+  const int64_t lim = side == 0 ? h->m : h->n;
+  for (int64_t i = 0; i < nrows; ++i)
+    if (rows[i] < 0 || rows[i] >= lim) {
+      cumf::set_last_error("cumf_debug_row_gram: row out of range");
+      return CUMF_ERR_INVALID;
+    }
+  if (nrows == 0) return CUMF_OK;
+  DeviceGuard dg(h->device);
+  const int D = cumf::dump_dim(h->f);
+  DevBuf drows, dout;
+  CUMF_TRY(drows.alloc(sizeof(int32_t) * nrows));
+  CUMF_TRY(dout.alloc(sizeof(float) * nrows * D * D));
+  CUMF_HIP_TRY(hipMemcpy(drows.p, rows, drows.bytes, hipMemcpyHostToDevice));
+  cumf::SweepArgs a{};
+  a.row_ptr = side == 0 ? h->rp.as<int32_t>() : h->cp.as<int32_t>();
+  a.col = side == 0 ? h->ci.as<int32_t>() : h->ri.as<int32_t>();
+  a.val = side == 0 ? h->rv.as<float>() : h->cv.as<float>();
+  a.theta = side == 0 ? h->T.as<float>() : h->X.as<float>();
+  a.ldf = h->ldf;
+  a.f = h->f;
+  a.lambda = h->lambda;
+  hipError_t e = cumf::launch_dump_kernel(a, drows.as<int32_t>(), nrows, dout.as<float>(), h->stream);
+  if (e != hipSuccess) {
+    cumf::set_last_error(std::string("dump kernel: ") + hipGetErrorString(e));
+    return CUMF_ERR_HIP;
+  }
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  CUMF_HIP_TRY(hipMemcpy(out, dout.p, dout.bytes, hipMemcpyDeviceToHost));
+  return CUMF_OK;
+}
+
+int cumf_debug_mfma_layout(int device, float* out) {
+  cumf::clear_last_error();
+  if (!out) return CUMF_ERR_INVALID;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    cumf::set_last_error("no such device");
+    return CUMF_ERR_NODEVICE;
+  }
+  DeviceGuard dg(device);
+  DevBuf d;
+  CUMF_TRY(d.alloc(sizeof(float) * 256));
+  CUMF_HIP_TRY(cumf::launch_mfma_layout(d.as<float>(), nullptr));
+  CUMF_HIP_TRY(hipDeviceSynchronize());
+  CUMF_HIP_TRY(hipMemcpy(out, d.p, sizeof(float) * 256, hipMemcpyDeviceToHost));
+  return CUMF_OK;
+}
+
+int cumf_debug_copy(int device, int64_t bytes, int iters, double* ms_per_launch) {
+  cumf::clear_last_error();
+  if (bytes <= 0 || iters <= 0 || !ms_per_launch) return CUMF_ERR_INVALID;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    cumf::set_last_error("no such device");
+    return CUMF_ERR_NODEVICE;
+  }
+  DeviceGuard dg(device);
+  bytes &= ~int64_t(15);
+  DevBuf src, dst;
+  CUMF_TRY(src.alloc(bytes));
+  CUMF_TRY(dst.alloc(bytes));
+  CUMF_HIP_TRY(hipMemset(src.p, 1, bytes));
+  hipEvent_t a, b;
+  CUMF_HIP_TRY(hipEventCreate(&a));
+  CUMF_HIP_TRY(hipEventCreate(&b));
+  CUMF_HIP_TRY(cumf::launch_copy(src.p, dst.p, bytes, nullptr));  // warm-up
+  CUMF_HIP_TRY(hipEventRecord(a, nullptr));
+  for (int i = 0; i < iters; ++i) CUMF_HIP_TRY(cumf::launch_copy(src.p, dst.p, bytes, nullptr));
+  CUMF_HIP_TRY(hipEventRecord(b, nullptr));
+  CUMF_HIP_TRY(hipEventSynchronize(b));
+  float ms = 0.f;
+  CUMF_HIP_TRY(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms_per_launch = ms / iters;
+  return CUMF_OK;
+}
+
+// Device-pointer half-sweep.  The CSR row pointers are read back to plan
+// the tasks; factor buffers with f % 4 != 0 are re-strided through padded
+// temporaries (the kernels read factor rows with 16-byte loads).
+int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const int32_t* d_col,
+                           const float* d_val, const float* d_theta, float* d_x, int64_t m,
+                           int64_t n, int f, int64_t nnz, float lambda, const cumf_options* opt) {
+  cumf::clear_last_error();
+  CUMF_TRY(check_f(f));
+  if (m <= 0 || n <= 0 || nnz < 0 || nnz > INT32_MAX || m > INT32_MAX || !d_row_ptr || !d_theta ||
+      !d_x || (nnz > 0 && (!d_col || !d_val)) || !(lambda >= 0.f)) {
+    cumf::set_last_error("cumf_half_sweep_device: bad arguments");
+    return CUMF_ERR_INVALID;
+  }
+  cumf_options o;
+  cumf_default_options(&o);
+  if (opt && opt->split > 0) o.split = opt->split;
+  std::vector<int32_t> rp(m + 1);
+  CUMF_HIP_TRY(hipMemcpyAsync(rp.data(), d_row_ptr, sizeof(int32_t) * (m + 1), hipMemcpyDeviceToHost, stream));
+  CUMF_HIP_TRY(hipStreamSynchronize(stream));
+  if (rp[0] != 0 || rp[m] != nnz) {
+    cumf::set_last_error("row pointers must start at 0 and end at nnz");
+    return CUMF_ERR_INVALID;
+  }
+  Plan plan;
+  CUMF_TRY(plan.build(rp.data(), m, o.split, f));
+  const int64_t ldf = round_up4(f);
+  DevBuf tpad, xpad, fail;
+  const float* theta = d_theta;
+  float* x = d_x;
+  if (ldf != f) {
+    CUMF_TRY(tpad.alloc(sizeof(float) * n * ldf));
+    CUMF_TRY(xpad.alloc(sizeof(float) * m * ldf));
+    CUMF_HIP_TRY(hipMemsetAsync(tpad.p, 0, tpad.bytes, stream));
+    CUMF_HIP_TRY(hipMemcpy2DAsync(tpad.p, sizeof(float) * ldf, d_theta, sizeof(float) * f,
+                                  sizeof(float) * f, n, hipMemcpyDeviceToDevice, stream));
+    theta = tpad.as<float>();
+    x = xpad.as<float>();
+  }
+  CUMF_TRY(fail.alloc(sizeof(int)));
+  CUMF_HIP_TRY(hipMemsetAsync(fail.p, 0, sizeof(int), stream));
+  cumf::SweepArgs a{};
+  a.row_ptr = d_row_ptr;
+  a.col = d_col;
+  a.val = d_val;
+  a.theta = theta;
+  a.x = x;
+  a.ldf = ldf;
+  a.tasks = plan.tasks.as<cumf::Task>();
+  a.ntasks = plan.ntasks;
+  a.heavy = plan.heavy.as<cumf::HeavyRow>();
+  a.nheavy = plan.nheavy;
+  a.partial = plan.partial.as<float>();
+  a.fail_counter = fail.as<int>();
+  a.f = f;
+  a.lambda = lambda;
+  CUMF_HIP_TRY(cumf::launch_task_kernel(a, stream));
+  CUMF_HIP_TRY(cumf::launch_finish_kernel(a, stream));
+  if (ldf != f)
+    CUMF_HIP_TRY(hipMemcpy2DAsync(d_x, sizeof(float) * f, x, sizeof(float) * ldf, sizeof(float) * f, m,
+                                  hipMemcpyDeviceToDevice, stream));
+  CUMF_HIP_TRY(hipStreamSynchronize(stream));
+  return CUMF_OK;
+}
+
+// Drop-in for als/src/cuda/als.cu:1000-1115 (see include/als.h).
+void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, float* csrVal,
+                  float* thetaTHost, float* XTHost, const int m, const int n, const int f,
+                  const long nnz, const float lambda, const int X_BATCH) {
+  cumf::clear_last_error();
+  auto run = [&]() -> int {
+    if (!csrRowIndex || !thetaTHost || !XTHost || m <= 0 || n <= 0 || nnz < 0 || X_BATCH < 1 ||
+        (nnz > 0 && (!csrColIndex || !csrVal))) {
+      cumf::set_last_error("doALSWithCSR: bad arguments");
+      return CUMF_ERR_INVALID;
+    }
+    CUMF_TRY(check_f(f));
+    DevBuf theta, x;
+    CUMF_TRY(theta.alloc(sizeof(float) * (size_t)n * f));
+    CUMF_TRY(x.alloc(sizeof(float) * (size_t)m * f));
+    CUMF_HIP_TRY(hipMemcpyAsync(theta.p, thetaTHost, theta.bytes, hipMemcpyHostToDevice, cuda_stream));
+    CUMF_TRY(cumf_half_sweep_device(cuda_stream, csrRowIndex, csrColIndex, csrVal, theta.as<float>(),
+                                    x.as<float>(), m, n, f, nnz, lambda, nullptr));
+    CUMF_HIP_TRY(hipMemcpyAsync(XTHost, x.p, x.bytes, hipMemcpyDeviceToHost, cuda_stream));
+    CUMF_HIP_TRY(hipStreamSynchronize(cuda_stream));
+    return CUMF_OK;
+  };
+  int rc = run();
+  std::string err = rc == CUMF_OK ? std::string() : std::string(cumf_last_error());
+  // The reference consumes the CSR arrays whatever happens to the sweep
+  // (als/src/cuda/als.cu:1058, 1101-1102).
+  (void)hipStreamSynchronize(cuda_stream);
+  if (csrVal) (void)freeBest(csrVal);
+  if (csrRowIndex) (void)freeBest(csrRowIndex);
+  if (csrColIndex) (void)freeBest(csrColIndex);
+  if (!err.empty()) cumf::set_last_error(err);
+}
+
+}  // extern "C"

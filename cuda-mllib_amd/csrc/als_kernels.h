@@ -1,0 +1,58 @@
+// als_kernels.h -- internal interface between the host driver and the
+// gfx950 kernels of the ALS half-sweep.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cumf {
+
+// One unit of work for a wavefront: `count` ratings of row `row`, starting
+// `start` ratings after the row's first.  slot < 0: the task covers the
+// whole row and solves it; slot >= 0: it writes a partial Gram to
+// partial[slot].
+struct Task {
+  int32_t row;
+  int32_t start;
+  int32_t count;
+  int32_t slot;
+};
+
+// A row whose Gram is the sum of `nchunks` partials starting at `slot`.
+struct HeavyRow {
+  int32_t row;
+  int32_t slot;
+  int32_t nchunks;
+  int32_t pad;
+};
+
+struct SweepArgs {
+  const int32_t* row_ptr;  // CSR of the side being solved (m+1)
+  const int32_t* col;      // column indices into theta (nnz)
+  const float* val;        // ratings (nnz)
+  const float* theta;      // factors of the other side, row stride ldf
+  float* x;                // factors being solved, row stride ldf
+  int64_t ldf;             // factor row stride in floats (multiple of 4, >= f)
+  const Task* tasks;
+  int64_t ntasks;
+  const HeavyRow* heavy;
+  int64_t nheavy;
+  float* partial;          // partial Grams, partial_floats_per_slot(f) each
+  int* fail_counter;       // rows whose factorisation met a non-positive pivot
+  int f;
+  float lambda;
+};
+
+int sweep_nb(int f);
+size_t partial_floats_per_slot(int f);
+int dump_dim(int f);
+hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s);
+hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s);
+hipError_t launch_dump_kernel(const SweepArgs& a, const int32_t* rows, int64_t nrows, float* out,
+                              hipStream_t s);
+hipError_t launch_rmse(const int32_t* u, const int32_t* v, const float* r, int64_t cnt, const float* X,
+                       const float* T, int64_t ldf, int f, double* partial, int nblocks, double* out,
+                       hipStream_t s);
+hipError_t launch_mfma_layout(float* out, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
+
+}  // namespace cumf

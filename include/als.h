@@ -1,0 +1,161 @@
+/*
+ * als.h -- C ABI of libGPUALS.so, the MI355X (gfx950) ALS update.
+ *
+ * Two layers:
+ *   1. doALSWithCSR: the drop-in for the reference's entry point of the
+ *      same name (IBMSparkGPU/CUDA-MLlib als/src/cuda/als.h:96-103,
+ *      definition als/src/cuda/als.cu:1000-1115).  Same name, C linkage,
+ *      parameter order, types and ownership; the stream is a hipStream_t
+ *      (an opaque pointer, like cudaStream_t).
+ *   2. cumf_*: a status-returning context API that keeps ratings and both
+ *      factor matrices resident in device memory and runs whole ALS
+ *      iterations, the held-out RMSE and per-kernel timing.  The reference
+ *      has no counterpart (its sweep loop and RMSE live in Spark); these
+ *      are what its Spark caller does around doALSWithCSR.
+ *
+ * No HIP or CUDA header is needed to include this file.
+ */
+#ifndef CUMF_ALS_H_
+#define CUMF_ALS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same definition as <hip/hip_runtime_api.h>; repeating a typedef is legal. */
+typedef struct ihipStream_t* hipStream_t;
+
+/* ------------------------------------------------------------------------
+ * 1. Drop-in entry point.
+ *
+ * One half-sweep: for every row u of the m x n CSR matrix R (0-based,
+ * int32 row pointers and column indices, fp32 values, all in DEVICE memory
+ * obtained from mallocBest), solve
+ *     (Theta_u^T Theta_u + lambda * n_u * I) x_u = Theta_u^T r_u
+ * where Theta_u are the rows of thetaTHost (HOST, n x f, row j at
+ * thetaTHost[j*f]) named by row u's column indices; write x_u to
+ * XTHost[u*f] (HOST, m x f).
+ *
+ * Matches als/src/cuda/als.cu:1000-1115:
+ *   - returns after the result is in XTHost (synchronous);
+ *   - CONSUMES the three CSR arrays: they are released with freeBest
+ *     (als/src/cuda/als.cu:1058, 1101-1102), so they must come from
+ *     mallocBest and must not be used by the caller afterwards;
+ *   - X_BATCH (>= 1) bounded the reference's temporary memory; the fused
+ *     kernels here need no per-row f x f buffer, so it is only validated.
+ * Differences (documented improvements):
+ *   - any 1 <= f <= 140 (the reference needs f % 10 == 0, als/README.md:63);
+ *   - rows with no ratings get x_u = 0 (the reference divides by zero);
+ *   - errors never exit(): the call returns early and cumf_last_error()
+ *     describes the failure (the CSR arrays are still released).
+ * ---------------------------------------------------------------------- */
+void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, float* csrVal,
+                  float* thetaTHost, float* XTHost, const int m, const int n, const int f,
+                  const long nnz, const float lambda, const int X_BATCH);
+
+/* ------------------------------------------------------------------------
+ * 2. Context API.
+ * ---------------------------------------------------------------------- */
+#define CUMF_OK 0
+#define CUMF_ERR_INVALID 1      /* bad argument */
+#define CUMF_ERR_HIP 2          /* HIP runtime error */
+#define CUMF_ERR_NOMEM 3        /* allocation failed */
+#define CUMF_ERR_NODEVICE 4     /* no usable GPU */
+#define CUMF_ERR_STATE 5        /* call out of order (e.g. no ratings yet) */
+#define CUMF_ERR_UNSUPPORTED 6  /* option combination not built */
+
+#define CUMF_OPTIONS_VERSION 1
+
+#define CUMF_PREC_FP32 0        /* FP32 Gram on MFMA, FP32 Cholesky */
+
+typedef struct cumf_options {
+  int32_t version;      /* CUMF_OPTIONS_VERSION */
+  int32_t precision;    /* CUMF_PREC_* */
+  int32_t split;        /* max ratings per wavefront task; longer rows are
+                           split and their partial Grams reduced (0: 2048) */
+  int32_t timing;       /* 1: time every kernel launch with HIP events */
+  int32_t reserved[12];
+} cumf_options;
+
+typedef struct cumf_stats {
+  double task_ms;          /* fused Gram + solve kernel (all launches) */
+  int64_t task_launches;
+  double finish_ms;        /* partial-Gram reduction + solve kernel */
+  int64_t finish_launches;
+  double rmse_ms;          /* held-out RMSE kernels */
+  int64_t rmse_launches;
+  double sweep_ms;         /* whole half-sweeps, first to last kernel */
+  int64_t sweeps;
+  int64_t failed_rows;     /* rows with a non-positive pivot (x_u set to 0) */
+  int64_t gram_flops;      /* 2 nnz f(f+1) summed over the half-sweeps run */
+} cumf_stats;
+
+typedef struct cumf_handle cumf_handle;
+
+int cumf_default_options(cumf_options* opt);
+/* device < 0: pick with get_gpu().  f in [1, 140], lambda >= 0. */
+int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_handle** out);
+int cumf_destroy(cumf_handle* h);
+/* Host arrays; copied to the device.  The CSC (= CSR of R^T) is optional:
+ * when csc_col_ptr is NULL it is built from the CSR on the host. */
+int cumf_set_ratings(cumf_handle* h, int64_t m, int64_t n, int64_t nnz, const int32_t* csr_row_ptr,
+                     const int32_t* csr_col, const float* csr_val, const int32_t* csc_col_ptr,
+                     const int32_t* csc_row, const float* csc_val);
+/* Held-out ratings (COO, host arrays) used by cumf_heldout_rmse. */
+int cumf_set_heldout(cumf_handle* h, int64_t count, const int32_t* u, const int32_t* v,
+                     const float* r);
+/* Host factors, row-major m x f and n x f; either may be NULL. */
+int cumf_set_factors(cumf_handle* h, const float* X, const float* theta);
+int cumf_get_factors(cumf_handle* h, float* X, float* theta);
+/* Half-sweeps, asynchronous on the handle's stream. */
+int cumf_update_x(cumf_handle* h);      /* X from Theta over the CSR */
+int cumf_update_theta(cumf_handle* h);  /* Theta from X over the CSC */
+/* iters x (update_x, update_theta). */
+int cumf_iterate(cumf_handle* h, int iters);
+/* sqrt(mean((r - x_u . theta_v)^2)) over the held-out set, FP64; blocks. */
+int cumf_heldout_rmse(cumf_handle* h, double* rmse);
+int cumf_synchronize(cumf_handle* h);
+/* Synchronises, then reports accumulated statistics. */
+int cumf_get_stats(cumf_handle* h, cumf_stats* stats);
+int cumf_reset_stats(cumf_handle* h);
+hipStream_t cumf_get_stream(cumf_handle* h);
+/* Thread-local description of the last failure on this thread ("" if none). */
+const char* cumf_last_error(void);
+
+/* Device-resident half-sweep with caller-owned device buffers (nothing
+ * consumed, nothing copied): x (m x f) from theta (n x f) over the CSR.
+ * Asynchronous on `stream`; the workspace it needs is allocated and freed
+ * inside the call, which therefore synchronises `stream` before returning. */
+int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const int32_t* d_col,
+                           const float* d_val, const float* d_theta, float* d_x, int64_t m,
+                           int64_t n, int f, int64_t nnz, float lambda, const cumf_options* opt);
+
+/* ------------------------------------------------------------------------
+ * Test and measurement hooks (no counterpart in the reference).
+ * ---------------------------------------------------------------------- */
+/* Augmented Gram of the listed rows of the X side (side 0, over the CSR)
+ * or Theta side (side 1, over the CSC), ridge included, as dense D x D
+ * row-major FP32 (D = cumf_debug_gram_dim(f)); A is the leading f x f
+ * block, b sits in column round_up(f,4).  out: host, nrows * D * D. */
+int cumf_debug_gram_dim(int f);
+int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t nrows, float* out);
+/* Runs one v_mfma_f32_16x16x4_f32 with A[i][k] = 100 i + k, B[k][j] =
+ * 1000 k + j and returns the 64 x 4 result registers (host out[256]). */
+int cumf_debug_mfma_layout(int device, float* out);
+/* Times `iters` launches of a plain copy kernel over `bytes` bytes (for
+ * calibrating HBM byte counters); ms_per_launch receives the mean. */
+int cumf_debug_copy(int device, int64_t bytes, int iters, double* ms_per_launch);
+/* 1: mallocBest skips device memory and takes its mapped-host fallback
+ * (also set by the environment variable CUMF_FORCE_HOST_ALLOC=1). */
+int cumf_debug_force_host_alloc(int on);
+/* Host <-> device copies for callers without a HIP binding (blocking). */
+int cumf_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int cumf_memcpy_d2h(void* dst, const void* src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUMF_ALS_H_ */
