@@ -28,6 +28,7 @@ for p in (os.path.join(REPO, "cuda-mllib_amd"), REPO):
 
 METRIC = "ALS s/iteration & hermitian TFLOP/s at f=100, vs plain FP32 at equal RMSE"
 PEAK_FP32_MFMA_TFLOPS = 157.3   # MI355X dense FP32 matrix peak
+PEAK_BF16_MFMA_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 PEAK_HBM_GBS = 8000.0           # MI355X HBM3E
 
 
@@ -82,9 +83,19 @@ def main():
     ap.add_argument("--f", type=int, default=100)
     ap.add_argument("--lam", type=float, default=0.058)
     ap.add_argument("--split", type=int, default=0)
+    ap.add_argument("--precision", choices=["fp32", "bf16gram"], default="fp32")
+    ap.add_argument("--graph", action="store_true", help="replay each iteration as a captured hipGraph")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        from cumf import als as G
+        ms = G.copy_kernel_ms(args.calibrate_copy, iters=5)
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "ms_per_launch": ms,
+                          "GBps": 2 * args.calibrate_copy / ms / 1e6}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the ALS sweep is single-device here "
                                    "(sharding across GPUs is described, not built)", "n_gpus": args.gpus}))
@@ -99,7 +110,8 @@ def main():
         f"in {time.time() - t0:.1f}s")
     f, lam = args.f, args.lam
     theta0 = cumf.init_factors(ds.n, f, 0)
-    h = G.ALS(f, lam, device=0, timing=1, split=args.split)
+    prec = {"fp32": G.CUMF_PREC_FP32, "bf16gram": G.CUMF_PREC_BF16_GRAM}[args.precision]
+    h = G.ALS(f, lam, device=0, timing=1, split=args.split, precision=prec, use_graph=int(args.graph))
     h.set_dataset(ds)
     h.set_factors(theta=theta0)
     log("ratings and factors resident on device")
@@ -120,9 +132,14 @@ def main():
 
     s_per_iter = elapsed / args.steps
     gram_flops_iter = 2.0 * ds.nnz * f * (f + 1) * 2
-    task_ms = st["task_ms"] / max(st["task_launches"], 1)
-    gram_flops_launch = st["gram_flops"] / max(st["task_launches"], 1)
+    if st["task_launches"]:  # per-kernel events
+        kern_ms_total, launches, kname = st["task_ms"], st["task_launches"], "als_task_kernel (fused gram+solve)"
+    else:  # graph replays: whole iterations
+        kern_ms_total, launches, kname = st["iter_ms"], st["iters"], "one captured iteration (hipGraph)"
+    task_ms = kern_ms_total / max(launches, 1)
+    gram_flops_launch = st["gram_flops"] / max(launches, 1)
     achieved = gram_flops_launch / (task_ms * 1e-3) / 1e12
+    peak = PEAK_FP32_MFMA_TFLOPS if args.precision == "fp32" else PEAK_BF16_MFMA_TFLOPS
     rec = {
         "metric": METRIC,
         "value": s_per_iter,
@@ -134,20 +151,22 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32" if args.precision == "fp32" else "bf16 gram / fp32 solve",
         "data": "synthetic",
         "config": {"workload": args.workload, "m": ds.m, "n": ds.n, "nnz": ds.nnz, "f": f,
-                   "lambda": lam, "precision": "fp32 gram (MFMA) + fp32 cholesky",
+                   "lambda": lam, "precision": {"fp32": "fp32 gram (MFMA) + fp32 cholesky",
+                                                "bf16gram": "bf16 gram (MFMA, fp32 accumulate) + fp32 cholesky"}[args.precision],
+                   "graph": bool(args.graph),
                    "split": args.split or 2048},
         "ratings_per_s": ds.nnz / s_per_iter,
-        "hermitian_tflops": gram_flops_iter / (st["task_ms"] / args.steps * 1e-3) / 1e12,
+        "hermitian_tflops": gram_flops_iter / (kern_ms_total / args.steps * 1e-3) / 1e12,
         "heldout_rmse": rmse,
         "failed_rows": st["failed_rows"],
         "kernel_ms_per_iter": {"als_task_kernel": st["task_ms"] / args.steps,
                                "als_finish_kernel": st["finish_ms"] / args.steps},
-        "roofline": {"kernel": "als_task_kernel (fused gram+solve)", "bound": "mfma",
-                     "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": None,
+        "roofline": {"kernel": kname, "bound": "mfma",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
                      "avg_launch_ms": task_ms,
                      "flops_per_launch": gram_flops_launch},
     }

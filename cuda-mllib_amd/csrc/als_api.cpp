@@ -14,6 +14,7 @@
 // the longest start first (longest-processing-time-first scheduling).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -118,12 +119,12 @@ int check_f(int f) {
 
 // Kernel-time bookkeeping with HIP events.
 struct Timer {
-  enum Kind { TASK = 0, FINISH, RMSE, SWEEP, NKINDS };
+  enum Kind { TASK = 0, FINISH, RMSE, SWEEP, ITER, NKINDS };
   struct Pending { hipEvent_t a, b; int kind; };
   std::vector<hipEvent_t> pool;
   std::vector<Pending> pending;
-  double ms[NKINDS] = {0, 0, 0, 0};
-  int64_t count[NKINDS] = {0, 0, 0, 0};
+  double ms[NKINDS] = {};
+  int64_t count[NKINDS] = {};
 
   ~Timer() {
     for (auto& p : pending) { pool.push_back(p.a); pool.push_back(p.b); }
@@ -160,12 +161,22 @@ struct cumf_handle {
   int64_t m = 0, n = 0, nnz = 0, ntest = 0;
   DevBuf rp, ci, rv, cp, ri, cv;  // CSR and CSC
   DevBuf X, T;                    // factors, row stride ldf
+  DevBuf Xb, Tb;                  // BF16 shadow factors (precision mode), row stride ldb
+  DevBuf zero16;                  // 64 zero bytes
+  int64_t ldb = 0;
   DevBuf tu, tv, tr;              // held-out COO
   DevBuf rmse_partial, rmse_out, fail;
   Plan plan_x, plan_t;
   Timer timer;
   int64_t gram_flops = 0;
   bool have_ratings = false;
+  // One captured iteration (use_graph), rebuilt after cumf_set_ratings.
+  hipGraphExec_t iter_graph = nullptr;
+  bool capturing = false;
+  void drop_graph() {
+    if (iter_graph) (void)hipGraphExecDestroy(iter_graph);
+    iter_graph = nullptr;
+  }
 };
 
 namespace {
@@ -187,7 +198,7 @@ struct DeviceGuard {
 template <class F>
 int timed(cumf_handle* h, int kind, F&& launch) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (h->opt.timing) {
+  if (h->opt.timing && !h->capturing) {
     a = h->timer.get();
     b = h->timer.get();
     if (a) (void)hipEventRecord(a, h->stream);
@@ -207,7 +218,22 @@ int timed(cumf_handle* h, int kind, F&& launch) {
 }
 
 int run_half_sweep(cumf_handle* h, const Plan& p, const DevBuf& rowp, const DevBuf& col,
-                   const DevBuf& val, const float* theta, float* x) {
+                   const DevBuf& val, const float* theta, float* x, const DevBuf& theta_b, DevBuf& x_b) {
+  if (h->opt.compat_hermitian_bug) {
+    cumf::CompatArgs c{};
+    c.row_ptr = rowp.as<int32_t>();
+    c.col = col.as<int32_t>();
+    c.val = val.as<float>();
+    c.theta = theta;
+    c.x = x;
+    c.ldf = h->ldf;
+    c.fail_counter = h->fail.as<int>();
+    c.f = h->f;
+    c.lambda = h->lambda;
+    CUMF_TRY(timed(h, Timer::TASK, [&] { return cumf::launch_compat(c, p.rows, h->stream); }));
+    h->gram_flops += 2 * p.nnz * (int64_t)h->f * (h->f + 1);
+    return CUMF_OK;
+  }
   cumf::SweepArgs a{};
   a.row_ptr = rowp.as<int32_t>();
   a.col = col.as<int32_t>();
@@ -223,8 +249,13 @@ int run_half_sweep(cumf_handle* h, const Plan& p, const DevBuf& rowp, const DevB
   a.fail_counter = h->fail.as<int>();
   a.f = h->f;
   a.lambda = h->lambda;
+  a.precision = h->opt.precision;
+  a.ldb = h->ldb;
+  a.theta_b = theta_b.as<uint16_t>();
+  a.x_b = x_b.as<uint16_t>();
+  a.zero16 = h->zero16.p;
   hipEvent_t s0 = nullptr, s1 = nullptr;
-  if (h->opt.timing) {
+  if (h->opt.timing && !h->capturing) {
     s0 = h->timer.get();
     s1 = h->timer.get();
     if (s0) (void)hipEventRecord(s0, h->stream);
@@ -240,10 +271,12 @@ int run_half_sweep(cumf_handle* h, const Plan& p, const DevBuf& rowp, const DevB
   return CUMF_OK;
 }
 
-int upload_factors(cumf_handle* h, DevBuf& dst, const float* src, int64_t rows) {
+int upload_factors(cumf_handle* h, DevBuf& dst, DevBuf& dst_b, const float* src, int64_t rows) {
   CUMF_HIP_TRY(hipMemsetAsync(dst.p, 0, dst.bytes, h->stream));
   CUMF_HIP_TRY(hipMemcpy2DAsync(dst.p, sizeof(float) * h->ldf, src, sizeof(float) * h->f,
                                 sizeof(float) * h->f, rows, hipMemcpyHostToDevice, h->stream));
+  if (h->opt.precision != CUMF_PREC_FP32)
+    CUMF_HIP_TRY(cumf::launch_to_bf16(dst.as<float>(), h->ldf, dst_b.p, h->ldb, rows, h->f, h->stream));
   return CUMF_OK;
 }
 
@@ -280,9 +313,17 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
     o = *opt;
     if (o.split <= 0) o.split = kDefaultSplit;
   }
-  if (o.precision != CUMF_PREC_FP32) {
+  if (o.precision != CUMF_PREC_FP32 && o.precision != CUMF_PREC_BF16_GRAM) {
     cumf::set_last_error("precision mode not built");
     return CUMF_ERR_UNSUPPORTED;
+  }
+  if (o.precision != CUMF_PREC_FP32 && o.compat_hermitian_bug) {
+    cumf::set_last_error("compat_hermitian_bug reproduces the FP32 reference; use CUMF_PREC_FP32");
+    return CUMF_ERR_INVALID;
+  }
+  if (o.compat_hermitian_bug && (f % 10 != 0 || f < 10)) {
+    cumf::set_last_error("compat_hermitian_bug reproduces the reference, which needs f % 10 == 0");
+    return CUMF_ERR_INVALID;
   }
   if (device < 0) device = get_gpu();
   int count = 0;
@@ -292,10 +333,16 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
     return CUMF_ERR_NODEVICE;
   }
   DeviceGuard dg(device);
+  if (o.compat_hermitian_bug && cumf::prepare_compat(f) != hipSuccess) {
+    (void)hipGetLastError();
+    cumf::set_last_error("compat kernel: cannot raise its LDS limit");
+    return CUMF_ERR_HIP;
+  }
   auto* h = new cumf_handle();
   h->device = device;
   h->f = f;
   h->ldf = round_up4(f);
+  h->ldb = (f + 7) & ~7;
   h->lambda = lambda;
   h->opt = o;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -307,6 +354,8 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
   int rc = h->fail.alloc(sizeof(int));
   if (rc == CUMF_OK) rc = h->rmse_partial.alloc(sizeof(double) * kRmseBlocks);
   if (rc == CUMF_OK) rc = h->rmse_out.alloc(sizeof(double));
+  if (rc == CUMF_OK) rc = h->zero16.alloc(64);
+  if (rc == CUMF_OK && hipMemset(h->zero16.p, 0, 64) != hipSuccess) rc = CUMF_ERR_HIP;
   if (rc == CUMF_OK && hipMemset(h->fail.p, 0, sizeof(int)) != hipSuccess) rc = CUMF_ERR_HIP;
   if (rc != CUMF_OK) {
     (void)hipStreamDestroy(h->stream);
@@ -323,6 +372,7 @@ int cumf_destroy(cumf_handle* h) {
     DeviceGuard dg(h->device);
     (void)hipStreamSynchronize(h->stream);
     h->timer.resolve();
+    h->drop_graph();
     (void)hipStreamDestroy(h->stream);
     h->stream = nullptr;
     delete h;  // DevBuf destructors free on the handle's device
@@ -348,26 +398,7 @@ int cumf_set_ratings(cumf_handle* h, int64_t m, int64_t n, int64_t nnz, const in
       cumf::set_last_error("cumf_set_ratings: column index out of range at " + std::to_string(k));
       return CUMF_ERR_INVALID;
     }
-  // CSC: given, or built here.
-  std::vector<int32_t> cp_b, ri_b;
-  std::vector<float> cv_b;
-  if (!csc_col_ptr) {
-    cp_b.assign(n + 1, 0);
-    for (int64_t k = 0; k < nnz; ++k) cp_b[csr_col[k] + 1]++;
-    for (int64_t j = 0; j < n; ++j) cp_b[j + 1] += cp_b[j];
-    std::vector<int64_t> pos(cp_b.begin(), cp_b.end() - 1);
-    ri_b.resize(nnz);
-    cv_b.resize(nnz);
-    for (int64_t u = 0; u < m; ++u)
-      for (int64_t k = csr_row_ptr[u]; k < csr_row_ptr[u + 1]; ++k) {
-        int64_t d = pos[csr_col[k]]++;
-        ri_b[d] = (int32_t)u;
-        cv_b[d] = csr_val[k];
-      }
-    csc_col_ptr = cp_b.data();
-    csc_row = ri_b.data();
-    csc_val = cv_b.data();
-  } else {
+  if (csc_col_ptr) {
     if (nnz > 0 && (!csc_row || !csc_val)) {
       cumf::set_last_error("cumf_set_ratings: CSC arrays missing");
       return CUMF_ERR_INVALID;
@@ -385,6 +416,7 @@ int cumf_set_ratings(cumf_handle* h, int64_t m, int64_t n, int64_t nnz, const in
   DeviceGuard dg(h->device);
   CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
   h->have_ratings = false;
+  h->drop_graph();
   CUMF_TRY(h->rp.alloc(sizeof(int32_t) * (m + 1)));
   CUMF_TRY(h->ci.alloc(sizeof(int32_t) * nnz));
   CUMF_TRY(h->rv.alloc(sizeof(float) * nnz));
@@ -393,14 +425,33 @@ int cumf_set_ratings(cumf_handle* h, int64_t m, int64_t n, int64_t nnz, const in
   CUMF_TRY(h->cv.alloc(sizeof(float) * nnz));
   CUMF_TRY(h->X.alloc(sizeof(float) * m * h->ldf));
   CUMF_TRY(h->T.alloc(sizeof(float) * n * h->ldf));
+  if (h->opt.precision != CUMF_PREC_FP32) {
+    CUMF_TRY(h->Xb.alloc(sizeof(uint16_t) * m * h->ldb));
+    CUMF_TRY(h->Tb.alloc(sizeof(uint16_t) * n * h->ldb));
+    CUMF_HIP_TRY(hipMemset(h->Xb.p, 0, h->Xb.bytes));
+    CUMF_HIP_TRY(hipMemset(h->Tb.p, 0, h->Tb.bytes));
+  }
   CUMF_HIP_TRY(hipMemcpy(h->rp.p, csr_row_ptr, h->rp.bytes, hipMemcpyHostToDevice));
   if (nnz > 0) {
     CUMF_HIP_TRY(hipMemcpy(h->ci.p, csr_col, h->ci.bytes, hipMemcpyHostToDevice));
     CUMF_HIP_TRY(hipMemcpy(h->rv.p, csr_val, h->rv.bytes, hipMemcpyHostToDevice));
-    CUMF_HIP_TRY(hipMemcpy(h->ri.p, csc_row, h->ri.bytes, hipMemcpyHostToDevice));
-    CUMF_HIP_TRY(hipMemcpy(h->cv.p, csc_val, h->cv.bytes, hipMemcpyHostToDevice));
   }
-  CUMF_HIP_TRY(hipMemcpy(h->cp.p, csc_col_ptr, h->cp.bytes, hipMemcpyHostToDevice));
+  std::vector<int32_t> cp_host;
+  if (csc_col_ptr) {
+    CUMF_HIP_TRY(hipMemcpy(h->cp.p, csc_col_ptr, h->cp.bytes, hipMemcpyHostToDevice));
+    if (nnz > 0) {
+      CUMF_HIP_TRY(hipMemcpy(h->ri.p, csc_row, h->ri.bytes, hipMemcpyHostToDevice));
+      CUMF_HIP_TRY(hipMemcpy(h->cv.p, csc_val, h->cv.bytes, hipMemcpyHostToDevice));
+    }
+  } else {
+    // Stable device transpose; the column pointers come back for planning.
+    CUMF_TRY(cumf_csr_to_csc_device(h->stream, m, n, nnz, h->rp.as<int32_t>(), h->ci.as<int32_t>(),
+                                    h->rv.as<float>(), h->cp.as<int32_t>(), h->ri.as<int32_t>(),
+                                    h->cv.as<float>()));
+    cp_host.resize(n + 1);
+    CUMF_HIP_TRY(hipMemcpy(cp_host.data(), h->cp.p, h->cp.bytes, hipMemcpyDeviceToHost));
+    csc_col_ptr = cp_host.data();
+  }
   CUMF_HIP_TRY(hipMemset(h->X.p, 0, h->X.bytes));
   CUMF_HIP_TRY(hipMemset(h->T.p, 0, h->T.bytes));
   CUMF_TRY(h->plan_x.build(csr_row_ptr, m, h->opt.split, h->f));
@@ -445,8 +496,8 @@ int cumf_set_factors(cumf_handle* h, const float* X, const float* theta) {
     return CUMF_ERR_STATE;
   }
   DeviceGuard dg(h->device);
-  if (X) CUMF_TRY(upload_factors(h, h->X, X, h->m));
-  if (theta) CUMF_TRY(upload_factors(h, h->T, theta, h->n));
+  if (X) CUMF_TRY(upload_factors(h, h->X, h->Xb, X, h->m));
+  if (theta) CUMF_TRY(upload_factors(h, h->T, h->Tb, theta, h->n));
   CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
   return CUMF_OK;
 }
@@ -474,7 +525,7 @@ int cumf_update_x(cumf_handle* h) {
     return CUMF_ERR_STATE;
   }
   DeviceGuard dg(h->device);
-  return run_half_sweep(h, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>());
+  return run_half_sweep(h, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb, h->Xb);
 }
 
 int cumf_update_theta(cumf_handle* h) {
@@ -483,13 +534,62 @@ int cumf_update_theta(cumf_handle* h) {
     return CUMF_ERR_STATE;
   }
   DeviceGuard dg(h->device);
-  return run_half_sweep(h, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>());
+  return run_half_sweep(h, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb, h->Tb);
 }
 
+namespace {
+
+// Captures one iteration (both half-sweeps: four kernel launches, no
+// memsets or host work) on the handle's stream into an executable graph.
+int capture_iteration(cumf_handle* h) {
+  h->drop_graph();
+  hipGraph_t g = nullptr;
+  CUMF_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  h->capturing = true;
+  const int64_t flops = h->gram_flops;
+  int rc = run_half_sweep(h, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb, h->Xb);
+  if (rc == CUMF_OK) rc = run_half_sweep(h, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb, h->Tb);
+  h->capturing = false;
+  h->gram_flops = flops;  // counted again at each replay
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc != CUMF_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    cumf::set_last_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return CUMF_ERR_HIP;
+  }
+  e = hipGraphInstantiate(&h->iter_graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    h->iter_graph = nullptr;
+    cumf::set_last_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return CUMF_ERR_HIP;
+  }
+  return CUMF_OK;
+}
+
+}  // namespace
+
 int cumf_iterate(cumf_handle* h, int iters) {
+  if (!h || !h->have_ratings) {
+    cumf::set_last_error("cumf_iterate: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  if (iters < 0) return CUMF_ERR_INVALID;
+  if (!h->opt.use_graph) {
+    for (int it = 0; it < iters; ++it) {
+      CUMF_TRY(cumf_update_x(h));
+      CUMF_TRY(cumf_update_theta(h));
+    }
+    return CUMF_OK;
+  }
+  DeviceGuard dg(h->device);
+  if (!h->iter_graph) CUMF_TRY(capture_iteration(h));
   for (int it = 0; it < iters; ++it) {
-    CUMF_TRY(cumf_update_x(h));
-    CUMF_TRY(cumf_update_theta(h));
+    CUMF_TRY(timed(h, Timer::ITER, [&] { return hipGraphLaunch(h->iter_graph, h->stream); }));
+    h->gram_flops += 2 * (h->plan_x.nnz + h->plan_t.nnz) * (int64_t)h->f * (h->f + 1);
   }
   return CUMF_OK;
 }
@@ -543,6 +643,8 @@ int cumf_get_stats(cumf_handle* h, cumf_stats* s) {
   s->sweeps = h->timer.count[Timer::SWEEP];
   s->failed_rows = fails;
   s->gram_flops = h->gram_flops;
+  s->iter_ms = h->timer.ms[Timer::ITER];
+  s->iters = h->timer.count[Timer::ITER];
   return CUMF_OK;
 }
 
@@ -590,11 +692,56 @@ int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t n
   a.ldf = h->ldf;
   a.f = h->f;
   a.lambda = h->lambda;
+  a.precision = h->opt.precision;
+  a.ldb = h->ldb;
+  a.theta_b = side == 0 ? h->Tb.as<uint16_t>() : h->Xb.as<uint16_t>();
+  a.zero16 = h->zero16.p;
   hipError_t e = cumf::launch_dump_kernel(a, drows.as<int32_t>(), nrows, dout.as<float>(), h->stream);
   if (e != hipSuccess) {
     cumf::set_last_error(std::string("dump kernel: ") + hipGetErrorString(e));
     return CUMF_ERR_HIP;
   }
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  CUMF_HIP_TRY(hipMemcpy(out, dout.p, dout.bytes, hipMemcpyDeviceToHost));
+  return CUMF_OK;
+}
+
+int cumf_debug_compat_system(cumf_handle* h, int side, const int32_t* rows, int64_t nrows, double* out) {
+  cumf::clear_last_error();
+  if (!h || !h->have_ratings || (side != 0 && side != 1) || nrows < 0 || (nrows > 0 && (!rows || !out)) ||
+      h->f % 10 != 0) {
+    cumf::set_last_error("cumf_debug_compat_system: bad arguments (f must be a multiple of 10)");
+    return CUMF_ERR_INVALID;
+  }
+  const int64_t lim = side == 0 ? h->m : h->n;
+  for (int64_t i = 0; i < nrows; ++i)
+    if (rows[i] < 0 || rows[i] >= lim) {
+      cumf::set_last_error("cumf_debug_compat_system: row out of range");
+      return CUMF_ERR_INVALID;
+    }
+  if (nrows == 0) return CUMF_OK;
+  DeviceGuard dg(h->device);
+  if (cumf::prepare_compat(h->f) != hipSuccess) {
+    (void)hipGetLastError();
+    cumf::set_last_error("compat kernel: cannot raise its LDS limit");
+    return CUMF_ERR_HIP;
+  }
+  DevBuf drows, dout;
+  CUMF_TRY(drows.alloc(sizeof(int32_t) * nrows));
+  CUMF_TRY(dout.alloc(sizeof(double) * nrows * h->f * (h->f + 1)));
+  CUMF_HIP_TRY(hipMemcpy(drows.p, rows, drows.bytes, hipMemcpyHostToDevice));
+  cumf::CompatArgs c{};
+  c.row_ptr = side == 0 ? h->rp.as<int32_t>() : h->cp.as<int32_t>();
+  c.col = side == 0 ? h->ci.as<int32_t>() : h->ri.as<int32_t>();
+  c.val = side == 0 ? h->rv.as<float>() : h->cv.as<float>();
+  c.theta = side == 0 ? h->T.as<float>() : h->X.as<float>();
+  c.ldf = h->ldf;
+  c.rows = drows.as<int32_t>();
+  c.dump = dout.as<double>();
+  c.fail_counter = h->fail.as<int>();
+  c.f = h->f;
+  c.lambda = h->lambda;
+  CUMF_HIP_TRY(cumf::launch_compat(c, nrows, h->stream));
   CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
   CUMF_HIP_TRY(hipMemcpy(out, dout.p, dout.bytes, hipMemcpyDeviceToHost));
   return CUMF_OK;
@@ -613,6 +760,24 @@ int cumf_debug_mfma_layout(int device, float* out) {
   DevBuf d;
   CUMF_TRY(d.alloc(sizeof(float) * 256));
   CUMF_HIP_TRY(cumf::launch_mfma_layout(d.as<float>(), nullptr));
+  CUMF_HIP_TRY(hipDeviceSynchronize());
+  CUMF_HIP_TRY(hipMemcpy(out, d.p, sizeof(float) * 256, hipMemcpyDeviceToHost));
+  return CUMF_OK;
+}
+
+int cumf_debug_mfma_bf16_layout(int device, float* out) {
+  cumf::clear_last_error();
+  if (!out) return CUMF_ERR_INVALID;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    cumf::set_last_error("no such device");
+    return CUMF_ERR_NODEVICE;
+  }
+  DeviceGuard dg(device);
+  DevBuf d;
+  CUMF_TRY(d.alloc(sizeof(float) * 256));
+  CUMF_HIP_TRY(cumf::launch_mfma_bf16_layout(d.as<float>(), nullptr));
   CUMF_HIP_TRY(hipDeviceSynchronize());
   CUMF_HIP_TRY(hipMemcpy(out, d.p, sizeof(float) * 256, hipMemcpyDeviceToHost));
   return CUMF_OK;
@@ -665,6 +830,11 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
   cumf_options o;
   cumf_default_options(&o);
   if (opt && opt->split > 0) o.split = opt->split;
+  if (opt) o.compat_hermitian_bug = opt->compat_hermitian_bug;
+  if (o.compat_hermitian_bug && f % 10 != 0) {
+    cumf::set_last_error("compat_hermitian_bug reproduces the reference, which needs f % 10 == 0");
+    return CUMF_ERR_INVALID;
+  }
   std::vector<int32_t> rp(m + 1);
   CUMF_HIP_TRY(hipMemcpyAsync(rp.data(), d_row_ptr, sizeof(int32_t) * (m + 1), hipMemcpyDeviceToHost, stream));
   CUMF_HIP_TRY(hipStreamSynchronize(stream));
@@ -704,8 +874,23 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
   a.fail_counter = fail.as<int>();
   a.f = f;
   a.lambda = lambda;
-  CUMF_HIP_TRY(cumf::launch_task_kernel(a, stream));
-  CUMF_HIP_TRY(cumf::launch_finish_kernel(a, stream));
+  if (o.compat_hermitian_bug) {
+    cumf::CompatArgs c{};
+    c.row_ptr = d_row_ptr;
+    c.col = d_col;
+    c.val = d_val;
+    c.theta = theta;
+    c.x = x;
+    c.ldf = ldf;
+    c.fail_counter = fail.as<int>();
+    c.f = f;
+    c.lambda = lambda;
+    CUMF_HIP_TRY(cumf::prepare_compat(f));
+    CUMF_HIP_TRY(cumf::launch_compat(c, m, stream));
+  } else {
+    CUMF_HIP_TRY(cumf::launch_task_kernel(a, stream));
+    CUMF_HIP_TRY(cumf::launch_finish_kernel(a, stream));
+  }
   if (ldf != f)
     CUMF_HIP_TRY(hipMemcpy2DAsync(d_x, sizeof(float) * f, x, sizeof(float) * ldf, sizeof(float) * f, m,
                                   hipMemcpyDeviceToDevice, stream));
@@ -729,8 +914,12 @@ void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, f
     CUMF_TRY(theta.alloc(sizeof(float) * (size_t)n * f));
     CUMF_TRY(x.alloc(sizeof(float) * (size_t)m * f));
     CUMF_HIP_TRY(hipMemcpyAsync(theta.p, thetaTHost, theta.bytes, hipMemcpyHostToDevice, cuda_stream));
+    cumf_options o;
+    cumf_default_options(&o);
+    const char* env = std::getenv("CUMF_COMPAT_HERMITIAN_BUG");
+    o.compat_hermitian_bug = (env && std::strcmp(env, "1") == 0) ? 1 : 0;
     CUMF_TRY(cumf_half_sweep_device(cuda_stream, csrRowIndex, csrColIndex, csrVal, theta.as<float>(),
-                                    x.as<float>(), m, n, f, nnz, lambda, nullptr));
+                                    x.as<float>(), m, n, f, nnz, lambda, &o));
     CUMF_HIP_TRY(hipMemcpyAsync(XTHost, x.p, x.bytes, hipMemcpyDeviceToHost, cuda_stream));
     CUMF_HIP_TRY(hipStreamSynchronize(cuda_stream));
     return CUMF_OK;

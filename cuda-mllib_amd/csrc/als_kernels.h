@@ -40,7 +40,31 @@ struct SweepArgs {
   int* fail_counter;       // rows whose factorisation met a non-positive pivot
   int f;
   float lambda;
+  // precision != 0: Gram from BF16 shadow factors (theta_b, row stride ldb,
+  // zero padded); solutions are also written to x_b in BF16.
+  int precision;
+  int64_t ldb;
+  const uint16_t* theta_b;
+  uint16_t* x_b;
+  const void* zero16;      // 16 zero bytes (gather source for padding)
 };
+
+// The reference-reproducing path (compat.hip): one work-group per row.
+struct CompatArgs {
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const float* val;
+  const float* theta;      // row stride ldf
+  float* x;                // row stride ldf
+  int64_t ldf;
+  const int32_t* rows;     // optional list of rows (NULL: rows 0..nrows-1)
+  double* dump;            // optional: write [M | b] (f*(f+1) doubles per row) and stop
+  int* fail_counter;       // rows with a zero pivot
+  int f;
+  float lambda;
+};
+hipError_t prepare_compat(int f);
+hipError_t launch_compat(const CompatArgs& a, int64_t nrows, hipStream_t s);
 
 int sweep_nb(int f);
 size_t partial_floats_per_slot(int f);
@@ -53,6 +77,9 @@ hipError_t launch_rmse(const int32_t* u, const int32_t* v, const float* r, int64
                        const float* T, int64_t ldf, int f, double* partial, int nblocks, double* out,
                        hipStream_t s);
 hipError_t launch_mfma_layout(float* out, hipStream_t s);
+hipError_t launch_mfma_bf16_layout(float* out, hipStream_t s);
+hipError_t launch_to_bf16(const float* src, int64_t ldf, void* dst, int64_t ldb, int64_t rows, int f,
+                          hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
 }  // namespace cumf

@@ -34,117 +34,224 @@
 namespace cumf {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) bf4 lds_bf4_t;
 
 constexpr int KB = 16;  // ratings per staged chunk
+
+// Waves per SIMD the fused task kernel is register-budgeted for.
+#ifndef CUMF_TASK_WAVES
+#define CUMF_TASK_WAVES 2
+#endif
 
 __host__ __device__ constexpr int num_tiles(int NB) { return NB * (NB + 1) / 2; }
 __host__ __device__ constexpr int tile_index(int I, int J, int NB) {
   return I * NB - I * (I - 1) / 2 + (J - I);
 }
-// LDS row stride of the staged chunk: >= 16*NB and an odd multiple of 16
-// floats, so the four 16-lane groups of a wavefront read four different
-// 16-bank windows.
-__host__ __device__ constexpr int stage_ld(int NB) { return 16 * NB + ((NB % 2 == 0) ? 16 : 0); }
 
-template <int NB>
-__host__ __device__ constexpr int lds_floats(int f) {
-  return (KB * stage_ld(NB) > f * (f + 1) / 2 + 16 * NB) ? KB * stage_ld(NB)
-                                                          : f * (f + 1) / 2 + 16 * NB;
+// LDS floats per wave: the staging buffers of the Gram phase and the
+// row/column buffers of the solve share the same space.
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int lds_floats_nb(int NB, int f) {
+  return cmax(cmax(2 * KB * ((f + 3) & ~3) + 2 * KB,   // FP32 staging
+                   2 * 32 * 16 * NB / 2),              // BF16 staging (2 x 32 rows x 16NB bf16)
+              cmax(f * (f + 1) / 2 + 2 * f, 16 * NB));  // U, z, z2; pivot-row buffer
 }
 
-__device__ __forceinline__ void wave_sync() {
-  // Work-groups here are a single wavefront: this orders LDS traffic
-  // between lanes (and is a compiler fence).
-  __syncthreads();
-}
 
 __device__ __forceinline__ float readlane_f(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// Waits for every outstanding vector-memory operation of this wave,
+// including LDS-DMA loads (global_load_lds), whose completion the compiler
+// does not track; also a compiler memory barrier.
+__device__ __forceinline__ void wait_vm_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Orders LDS accesses of one wave (LDS executes a wave's DS instructions in
+// issue order, so only the compiler must be kept from reordering).
+__device__ __forceinline__ void lds_order() {
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // ---------------------------------------------------------------------------
 // Gram accumulation over a contiguous range of one row's ratings.
+//
+// Staging: two LDS buffers of KB factor rows (row-major, stride ldf) plus
+// KB ratings.  Factor rows are gathered straight into LDS with
+// global_load_lds_dwordx4 (lane l of load `it` moves 16 bytes of slot
+// it*64+l = (row, quad) to LDS offset 16*(it*64+l), i.e. row*ldf+4*quad),
+// one chunk ahead of the MFMAs.  Rows past the end of the range are
+// zeroed when the MFMA operands are read.
 // ---------------------------------------------------------------------------
 template <int NB>
 __device__ __forceinline__ void gram_range(f4 (&acc)[num_tiles(NB)], const int32_t* __restrict__ col,
                                            const float* __restrict__ val,
                                            const float* __restrict__ theta, int64_t ldf,
-                                           int64_t start, int count, float* __restrict__ stage,
+                                           int64_t start, int count, float* __restrict__ lds,
                                            int lane) {
-  constexpr int LD = stage_ld(NB);
   const int g = lane >> 4, c = lane & 15;
-  const int Q = (int)(ldf >> 2);        // float4 per factor row
-  const int slots = KB * Q;             // float4 per chunk
-  const int bcol = (int)ldf;
-
-  // Zero the staging tile once (padding columns stay zero afterwards).
-  for (int i = lane; i < KB * LD; i += 64) stage[i] = 0.0f;
-
-  // Per-lane (row, quad) of each load slot; NB slots per lane suffice
-  // because ldf < 16*NB.
-  int srow[NB], squad[NB];
-#pragma unroll
-  for (int it = 0; it < NB; ++it) {
-    int s = it * 64 + lane;
-    srow[it] = s < slots ? s / Q : KB;  // KB marks an idle slot
-    squad[it] = s < slots ? s % Q : 0;
-  }
-
+  const int ld = (int)ldf;
+  const int Q = ld >> 2;       // 16-byte quads per factor row
+  const int slots = KB * Q;    // quads per chunk
+  const int cf = KB * ld;      // floats per staged chunk
+  float* rb = lds + 2 * cf;    // ratings, [2][KB]
+  // Last column block: factor columns, then the rating column (ldf), then zeros.
+  const int lastc = 16 * (NB - 1) + c;
+  const int lastkind = lastc < ld ? 0 : (lastc == ld ? 1 : 2);
+  const int lastoff = lastc < ld ? lastc : ld - 1;
   const int nch = (count + KB - 1) / KB;
-  // Software pipeline: column indices/values two chunks ahead, factor rows
-  // one chunk ahead.
-  auto load_cv = [&](int ch, int& ci, float& cv) {
-    int64_t k = (int64_t)ch * KB + lane;
-    bool ok = lane < KB && ch < nch && k < count;
-    ci = ok ? col[start + k] : -1;
-    cv = ok ? val[start + k] : 0.0f;
+
+  auto load_ci = [&](int ch) -> int {
+    const int k = ch * KB + lane;
+    return (lane < KB && ch < nch && k < count) ? col[start + k] : 0;
   };
-  auto load_rows = [&](int ci, f4 (&th)[NB]) {
+  auto issue = [&](int ch, int ci) {
+    float* yb = lds + (ch & 1) * cf;
 #pragma unroll
     for (int it = 0; it < NB; ++it) {
-      int src = __shfl(ci, srow[it] < KB ? srow[it] : 0);
-      if (srow[it] < KB && src >= 0)
-        th[it] = *reinterpret_cast<const f4*>(theta + (int64_t)src * ldf + 4 * squad[it]);
-      else
-        th[it] = f4{0.f, 0.f, 0.f, 0.f};
+      if (it * 64 < slots) {
+        const int sl = it * 64 + lane;
+        const int row = sl / Q;
+        const int q = sl - row * Q;
+        const int src = __shfl(ci, row < KB ? row : 0);
+        if (sl < slots)
+          __builtin_amdgcn_global_load_lds(theta + (int64_t)src * ldf + 4 * q,
+                                           (lds_void_t*)(yb + it * 256), 16, 0, 0);
+      }
     }
+    const int k = ch * KB + lane;
+    if (lane < KB && k < count)
+      __builtin_amdgcn_global_load_lds(val + start + k, (lds_void_t*)(rb + (ch & 1) * KB), 4, 0, 0);
   };
 
-  int ci0, ci1;
-  float cv0, cv1;
-  f4 th[NB];
-  load_cv(0, ci0, cv0);
-  load_cv(1, ci1, cv1);
-  load_rows(ci0, th);
-
+  int ci_next = load_ci(0);
+  issue(0, ci_next);
+  ci_next = load_ci(1);
   for (int ch = 0; ch < nch; ++ch) {
-    wave_sync();  // previous chunk's LDS reads are done
+    wait_vm_all();  // chunk ch is in LDS; column indices of ch+1 are in ci_next
+    if (ch + 1 < nch) issue(ch + 1, ci_next);
+    ci_next = load_ci(ch + 2);
+    const float* yb = lds + (ch & 1) * cf;
+    const float* rr = rb + (ch & 1) * KB;
+    const int valid = count - ch * KB;
 #pragma unroll
-    for (int it = 0; it < NB; ++it)
-      if (srow[it] < KB) *reinterpret_cast<f4*>(stage + srow[it] * LD + 4 * squad[it]) = th[it];
-    if (lane < KB) stage[lane * LD + bcol] = cv0;
-    wave_sync();
-    // Next chunk in flight while this one is multiplied.
-    int ci2;
-    float cv2;
-    load_cv(ch + 2, ci2, cv2);
-    load_rows(ci1, th);
-    ci0 = ci1; cv0 = cv1; ci1 = ci2; cv1 = cv2;
-
-    float a[KB / 4][NB];
+    for (int s = 0; s < KB / 4; ++s) {
+      const int row = 4 * s + g;
+      const bool ok = row < valid;
+      float a[NB];
 #pragma unroll
-    for (int s = 0; s < KB / 4; ++s)
-#pragma unroll
-      for (int I = 0; I < NB; ++I) a[s][I] = stage[(4 * s + g) * LD + 16 * I + c];
-#pragma unroll
-    for (int s = 0; s < KB / 4; ++s)
+      for (int I = 0; I < NB - 1; ++I) a[I] = ok ? yb[row * ld + 16 * I + c] : 0.0f;
+      const float yl = yb[row * ld + lastoff];
+      const float rl = rr[row];
+      a[NB - 1] = !ok ? 0.0f : (lastkind == 0 ? yl : (lastkind == 1 ? rl : 0.0f));
 #pragma unroll
       for (int I = 0; I < NB; ++I)
 #pragma unroll
         for (int J = I; J < NB; ++J)
           acc[tile_index(I, J, NB)] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-              a[s][I], a[s][J], acc[tile_index(I, J, NB)], 0, 0, 0);
+              a[I], a[J], acc[tile_index(I, J, NB)], 0, 0, 0);
+    }
+    lds_order();
   }
+  wait_vm_all();
+}
+
+// ---------------------------------------------------------------------------
+// BF16 Gram (precision mode): the same augmented product on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+//
+// Factor rows come from a BF16 shadow copy (row stride ldb = round_up(f,8),
+// zero padded).  A chunk of 32 ratings is staged in LDS as 32 rows of
+// W = 16*NB bf16: 2*NB 16-byte slots per row, moved with
+// global_load_lds_dwordx4 (slots past the factor row, and every slot of a
+// row past the end of the range, come from a zero block).  The rating r is
+// written into columns f and f+1 as bf16(r) and bf16(r - bf16(r)), so
+// b = Y^T r keeps ~16 significant bits (exact for integer ratings).
+//
+// MFMA operands come straight from the row-major tile with
+// ds_read_b64_tr_b16: within a 16-lane group, lane j addresses 4 bf16 of
+// row (8*kg + j/4), columns 16I + 4*(j%4); the transpose read returns to
+// lane i the element (i%4) of the segment of lane 4s + i/4, i.e. column
+// 16I + i of rows 8kg + s, s = 0..3.  A second read 4 rows further gives
+// the 8 consecutive k values the 16x16x32 operand needs.
+// ---------------------------------------------------------------------------
+constexpr int KBB = 32;  // ratings per BF16 chunk (one MFMA k-step)
+
+template <int NB>
+__device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const int32_t* __restrict__ col,
+                                                const float* __restrict__ val,
+                                                const __bf16* __restrict__ theta_b, int64_t ldb,
+                                                const void* __restrict__ zero16, int f, int64_t start,
+                                                int count, __bf16* __restrict__ lds, int lane) {
+  constexpr int W = 16 * NB;        // bf16 per staged row
+  constexpr int SPR = 2 * NB;       // 16-byte slots per staged row
+  constexpr int CB = KBB * W;       // bf16 per staged chunk
+  const int real = (int)(ldb >> 3);  // slots carrying factor data
+  const int kg = lane >> 4, j = lane & 15;
+  const int nch = (count + KBB - 1) / KBB;
+  // this lane's transpose-read offsets inside a chunk (bf16 units)
+  const int troff = (8 * kg + (j >> 2)) * W + 4 * (j & 3);
+
+  auto load_ci = [&](int ch) -> int {
+    const int k = ch * KBB + lane;
+    return (lane < KBB && ch < nch && k < count) ? col[start + k] : -1;
+  };
+  auto load_r = [&](int ch) -> float {
+    const int k = ch * KBB + lane;
+    return (lane < KBB && ch < nch && k < count) ? val[start + k] : 0.0f;
+  };
+  auto issue = [&](int ch, int ci) {
+    __bf16* yb = lds + (ch & 1) * CB;
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+      const int sl = it * 64 + lane;  // NB*64 = KBB*SPR slots exactly
+      const int row = sl / SPR;
+      const int q = sl - row * SPR;
+      const int src = __shfl(ci, row);
+      const void* g = (src >= 0 && q < real) ? (const void*)(theta_b + (int64_t)src * ldb + 8 * q) : zero16;
+      __builtin_amdgcn_global_load_lds(g, (lds_void_t*)(yb + it * 512), 16, 0, 0);
+    }
+  };
+
+  int ci = load_ci(0);
+  float rv = load_r(0);
+  issue(0, ci);
+  ci = load_ci(1);
+  float rnext = load_r(1);
+  for (int ch = 0; ch < nch; ++ch) {
+    wait_vm_all();  // chunk ch staged; indices and ratings of ch+1 loaded
+    __bf16* yb = lds + (ch & 1) * CB;
+    if (lane < KBB && ch * KBB + lane < count) {
+      const __bf16 hi = (__bf16)rv;
+      const __bf16 lo = (__bf16)(rv - (float)hi);
+      yb[lane * W + f] = hi;
+      yb[lane * W + f + 1] = lo;
+    }
+    if (ch + 1 < nch) issue(ch + 1, ci);
+    rv = rnext;
+    ci = load_ci(ch + 2);
+    rnext = load_r(ch + 2);
+    lds_order();
+    bf8 op[NB];
+#pragma unroll
+    for (int I = 0; I < NB; ++I) {
+      const bf4 lo4 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf4_t*)(yb + troff + 16 * I));
+      const bf4 hi4 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf4_t*)(yb + troff + 4 * W + 16 * I));
+      op[I] = __builtin_shufflevector(lo4, hi4, 0, 1, 2, 3, 4, 5, 6, 7);
+    }
+#pragma unroll
+    for (int I = 0; I < NB; ++I)
+#pragma unroll
+      for (int J = I; J < NB; ++J)
+        acc[tile_index(I, J, NB)] =
+            __builtin_amdgcn_mfma_f32_16x16x32_bf16(op[I], op[J], acc[tile_index(I, J, NB)], 0, 0, 0);
+    lds_order();
+  }
+  wait_vm_all();
 }
 
 // Factorisation steps k in [16K, 16K+16) of block row K, then the next
@@ -162,29 +269,35 @@ __device__ __forceinline__ void chol_block(f4 (&acc)[num_tiles(NB)], int f, floa
       for (int kr = 0; kr < 4; ++kr) {
         const int k = 16 * K + 4 * kg + kr;
         if (k < f) {
+          // Row k of the trailing matrix, a_k*, from its 16 owner lanes.
           if (g == kg) {
 #pragma unroll
             for (int J = K; J < NB; ++J) rowbuf[16 * J + c] = acc[tile_index(K, J, NB)][kr];
           }
-          wave_sync();
+          lds_order();
           const float d = rowbuf[k];
-          float inv = 0.0f;
-          if (d > 0.0f && d < __builtin_huge_valf()) inv = 1.0f / sqrtf(d);
-          else ok = false;
-          float colv[NB];
-          float rowv[NB][4];
+          const bool good = d > 0.0f && d < __builtin_huge_valf();
+          ok = ok && good;
+          const float inv = good ? __builtin_amdgcn_rsqf(d) : 0.0f;  // 1/sqrt(d)
+          const float inv2 = inv * inv;                               // 1/d
+          // a_ij -= a_ki a_kj / d  for rows i > k; u_kj = a_kj / sqrt(d).
+          float cj[NB], uj[NB];
 #pragma unroll
-          for (int J = K; J < NB; ++J) colv[J] = rowbuf[16 * J + c] * inv;
-#pragma unroll
-          for (int I = K; I < NB; ++I) {
-            const f4 v = *reinterpret_cast<const f4*>(rowbuf + 16 * I + 4 * g);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) rowv[I][r] = (16 * I + 4 * g + r > k) ? v[r] * inv : 0.0f;
+          for (int J = K; J < NB; ++J) {
+            const float v = rowbuf[16 * J + c];
+            cj[J] = v * inv2;
+            uj[J] = v * inv;
           }
-          wave_sync();
+          f4 rv[NB];
+#pragma unroll
+          for (int I = K; I < NB; ++I) rv[I] = *reinterpret_cast<const f4*>(rowbuf + 16 * I + 4 * g);
+          // Rows of block K at or above k are finished; later blocks are all below k.
+#pragma unroll
+          for (int r = 0; r < 4; ++r) rv[K][r] = (4 * g + r > 4 * kg + kr) ? rv[K][r] : 0.0f;
+          lds_order();
           if (g == kg) {
 #pragma unroll
-            for (int J = K; J < NB; ++J) acc[tile_index(K, J, NB)][kr] = colv[J];
+            for (int J = K; J < NB; ++J) acc[tile_index(K, J, NB)][kr] = uj[J];
           }
 #pragma unroll
           for (int I = K; I < NB; ++I)
@@ -192,8 +305,7 @@ __device__ __forceinline__ void chol_block(f4 (&acc)[num_tiles(NB)], int f, floa
             for (int J = I; J < NB; ++J)
 #pragma unroll
               for (int r = 0; r < 4; ++r)
-                acc[tile_index(I, J, NB)][r] =
-                    fmaf(-rowv[I][r], colv[J], acc[tile_index(I, J, NB)][r]);
+                acc[tile_index(I, J, NB)][r] = fmaf(-rv[I][r], cj[J], acc[tile_index(I, J, NB)][r]);
         }
       }
     }
@@ -205,10 +317,14 @@ __device__ __forceinline__ void chol_block(f4 (&acc)[num_tiles(NB)], int f, floa
 // In-register Cholesky of the augmented Gram + back substitution.
 // Returns false if a pivot is not positive (x is then written as zeros).
 // ---------------------------------------------------------------------------
+// bcol2 >= 0: a second right-hand-side column whose solution is added to
+// the first (the BF16 low part of the ratings).  xb: optional BF16 copy of
+// the solution (row of ldb entries, zero padded).
 template <int NB>
 __device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int bcol,
                                            float* __restrict__ lds, float* __restrict__ xrow,
-                                           int64_t ldf, int lane) {
+                                           int64_t ldf, int lane, int bcol2 = -1,
+                                           __bf16* __restrict__ xb = nullptr, int64_t ldb = 0) {
   const int g = lane >> 4, c = lane & 15;
   bool ok = true;
   chol_block<NB, 0>(acc, f, lds, ok, lane);
@@ -217,7 +333,8 @@ __device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int 
   const int upk = f * (f + 1) / 2;
   float* ucol = lds;
   float* zbuf = lds + upk;
-  wave_sync();
+  float* zbuf2 = zbuf + f;
+  lds_order();
 #pragma unroll
   for (int I = 0; I < NB; ++I)
 #pragma unroll
@@ -230,16 +347,17 @@ __device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int 
         const float v = acc[tile_index(I, J, NB)][r];
         if (j < f && i <= j) ucol[cbase + i] = v;
         if (j == bcol && i < f) zbuf[i] = v;
+        if (j == bcol2 && i < f) zbuf2[i] = v;
       }
     }
-  wave_sync();
+  lds_order();
 
   // Back substitution, one row index per lane (f <= 192).
   float z[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int i = lane + 64 * q;
-    z[q] = (i < f) ? zbuf[i] : 0.0f;
+    z[q] = (i < f) ? zbuf[i] + (bcol2 >= 0 ? zbuf2[i] : 0.0f) : 0.0f;
   }
   for (int k = f - 1; k >= 0; --k) {
     const int kq = k >> 6, kl = k & 63;
@@ -257,7 +375,9 @@ __device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int 
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int i = lane + 64 * q;
-    if (i < ldf) xrow[i] = (ok && i < f) ? z[q] : 0.0f;
+    const float xv = (ok && i < f) ? z[q] : 0.0f;
+    if (i < ldf) xrow[i] = xv;
+    if (xb && i < ldb) xb[i] = (__bf16)xv;
   }
   return ok;
 }
@@ -275,8 +395,18 @@ __device__ __forceinline__ void add_ridge(f4 (&acc)[num_tiles(NB)], int f, float
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
-template <int NB>
-__global__ __launch_bounds__(64) void als_task_kernel(SweepArgs a) {
+template <int NB, bool BF>
+__device__ __forceinline__ void gram_any(f4 (&acc)[num_tiles(NB)], const SweepArgs& a, int64_t start, int count,
+                                         float* lds, int lane) {
+  if constexpr (BF)
+    gram_range_bf16<NB>(acc, a.col, a.val, reinterpret_cast<const __bf16*>(a.theta_b), a.ldb, a.zero16, a.f,
+                        start, count, reinterpret_cast<__bf16*>(lds), lane);
+  else
+    gram_range<NB>(acc, a.col, a.val, a.theta, a.ldf, start, count, lds, lane);
+}
+
+template <int NB, bool BF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_task_kernel(SweepArgs a) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x;
   const Task t = a.tasks[blockIdx.x];
@@ -285,7 +415,7 @@ __global__ __launch_bounds__(64) void als_task_kernel(SweepArgs a) {
   for (int i = 0; i < num_tiles(NB); ++i) acc[i] = f4{0.f, 0.f, 0.f, 0.f};
 
   const int64_t start = (int64_t)a.row_ptr[t.row] + t.start;
-  if (t.count > 0) gram_range<NB>(acc, a.col, a.val, a.theta, a.ldf, start, t.count, lds, lane);
+  if (t.count > 0) gram_any<NB, BF>(acc, a, start, t.count, lds, lane);
 
   if (t.slot >= 0) {  // partial Gram of a heavy row
     float* dst = a.partial + (int64_t)t.slot * num_tiles(NB) * 256;
@@ -296,14 +426,18 @@ __global__ __launch_bounds__(64) void als_task_kernel(SweepArgs a) {
     return;
   }
   float* xrow = a.x + (int64_t)t.row * a.ldf;
+  __bf16* xb = BF ? reinterpret_cast<__bf16*>(a.x_b) + (int64_t)t.row * a.ldb : nullptr;
   if (t.count == 0) {  // no ratings: x_u = 0
     for (int i = lane; i < a.ldf; i += 64) xrow[i] = 0.0f;
+    if (xb)
+      for (int i = lane; i < a.ldb; i += 64) xb[i] = (__bf16)0.0f;
     return;
   }
   add_ridge<NB>(acc, a.f, a.lambda * (float)t.count, lane);
-  wave_sync();
-  if (!chol_solve<NB>(acc, a.f, (int)a.ldf, lds, xrow, a.ldf, lane) && lane == 0)
-    atomicAdd(a.fail_counter, 1);
+  lds_order();
+  const bool ok = BF ? chol_solve<NB>(acc, a.f, a.f, lds, xrow, a.ldf, lane, a.f + 1, xb, a.ldb)
+                     : chol_solve<NB>(acc, a.f, (int)a.ldf, lds, xrow, a.ldf, lane);
+  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
 }
 
 template <int NB>
@@ -324,13 +458,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void al
   const int n_u = a.row_ptr[h.row + 1] - a.row_ptr[h.row];
   add_ridge<NB>(acc, a.f, a.lambda * (float)n_u, lane);
   float* xrow = a.x + (int64_t)h.row * a.ldf;
-  if (!chol_solve<NB>(acc, a.f, (int)a.ldf, lds, xrow, a.ldf, lane) && lane == 0)
-    atomicAdd(a.fail_counter, 1);
+  const bool bf = a.precision != 0;
+  __bf16* xb = bf ? reinterpret_cast<__bf16*>(a.x_b) + (int64_t)h.row * a.ldb : nullptr;
+  const bool ok = bf ? chol_solve<NB>(acc, a.f, a.f, lds, xrow, a.ldf, lane, a.f + 1, xb, a.ldb)
+                     : chol_solve<NB>(acc, a.f, (int)a.ldf, lds, xrow, a.ldf, lane);
+  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
 }
 
 // Gram dump for tests: the full augmented (16NB)^2 matrix of each listed
 // row (ridge included), row-major, upper tiles mirrored.
-template <int NB>
+template <int NB, bool BF>
 __global__ __launch_bounds__(64) void als_dump_kernel(SweepArgs a, const int32_t* rows, float* out) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
@@ -340,7 +477,7 @@ __global__ __launch_bounds__(64) void als_dump_kernel(SweepArgs a, const int32_t
   for (int i = 0; i < num_tiles(NB); ++i) acc[i] = f4{0.f, 0.f, 0.f, 0.f};
   const int64_t s = a.row_ptr[u];
   const int cnt = a.row_ptr[u + 1] - a.row_ptr[u];
-  if (cnt > 0) gram_range<NB>(acc, a.col, a.val, a.theta, a.ldf, s, cnt, lds, lane);
+  if (cnt > 0) gram_any<NB, BF>(acc, a, s, cnt, lds, lane);
   add_ridge<NB>(acc, a.f, a.lambda * (float)cnt, lane);
   const int D = 16 * NB;
   float* o = out + (int64_t)blockIdx.x * D * D;
@@ -414,6 +551,33 @@ __global__ __launch_bounds__(64) void mfma_layout_kernel(float* out) {
   for (int r = 0; r < 4; ++r) out[l * 4 + r] = acc[r];
 }
 
+// FP32 rows (stride ldf) -> BF16 rows (stride ldb, zero padded).
+__global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ src, int64_t ldf,
+                                                      __bf16* __restrict__ dst, int64_t ldb, int64_t rows, int f) {
+  const int64_t total = rows * ldb;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = k / ldb, c = k - r * ldb;
+    dst[k] = (__bf16)(c < f ? src[r * ldf + c] : 0.0f);
+  }
+}
+
+// One v_mfma_f32_16x16x32_bf16 with A[i][k] = i + k/32... (small integers,
+// exact in bf16) fed as lane l: A[l%16][8(l/16)+j]; writes the 64 x 4
+// result registers.  Pins the operand and result layout of the builtin.
+__global__ __launch_bounds__(64) void mfma_bf16_layout_kernel(float* out) {
+  const int l = threadIdx.x;
+  bf8 av, bv;
+  for (int t = 0; t < 8; ++t) {
+    const int k = 8 * (l >> 4) + t;
+    av[t] = (__bf16)(float)((l & 15) + 2 * (k & 3));   // A[i][k] = i + 2 (k mod 4)
+    bv[t] = (__bf16)(float)((k % 5) + 3 * (l & 15));   // B[k][j] = k mod 5 + 3 j
+  }
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[l * 4 + r] = acc[r];
+}
+
 // Plain device copy used to calibrate the HBM byte counters.
 __global__ __launch_bounds__(256) void copy_kernel(const f4* __restrict__ src, f4* __restrict__ dst,
                                                    int64_t n4) {
@@ -425,24 +589,35 @@ __global__ __launch_bounds__(256) void copy_kernel(const f4* __restrict__ src, f
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
+// 16-column blocks of the augmented tile grid: room for the FP32 layout
+// (factor columns to ldf, rating at ldf) and the BF16 layout (factor
+// columns to ldb = round_up(f,8), rating hi/lo at f, f+1).
 int sweep_nb(int f) {
-  const int ldf = (f + 3) & ~3;
-  return (ldf + 1 + 15) / 16;
+  const int ldf = (f + 3) & ~3, ldb = (f + 7) & ~7;
+  int w = ldf + 1;
+  if (ldb > w) w = ldb;
+  if (f + 2 > w) w = f + 2;
+  return (w + 15) / 16;
 }
 
 template <int NB>
 static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which, const int32_t* rows,
                             float* out, int64_t nrows) {
-  const size_t shmem = sizeof(float) * (size_t)lds_floats<NB>(a.f);
+  const size_t shmem = sizeof(float) * (size_t)lds_floats_nb(NB, a.f);
+  const bool bf = a.precision != 0;
   if (which == 0) {
-    if (a.ntasks > 0)
-      hipLaunchKernelGGL(als_task_kernel<NB>, dim3((unsigned)a.ntasks), dim3(64), shmem, s, a);
+    if (a.ntasks > 0) {
+      if (bf) hipLaunchKernelGGL((als_task_kernel<NB, true>), dim3((unsigned)a.ntasks), dim3(64), shmem, s, a);
+      else hipLaunchKernelGGL((als_task_kernel<NB, false>), dim3((unsigned)a.ntasks), dim3(64), shmem, s, a);
+    }
   } else if (which == 1) {
     if (a.nheavy > 0)
       hipLaunchKernelGGL(als_finish_kernel<NB>, dim3((unsigned)a.nheavy), dim3(64), shmem, s, a);
   } else {
-    if (nrows > 0)
-      hipLaunchKernelGGL(als_dump_kernel<NB>, dim3((unsigned)nrows), dim3(64), shmem, s, a, rows, out);
+    if (nrows > 0) {
+      if (bf) hipLaunchKernelGGL((als_dump_kernel<NB, true>), dim3((unsigned)nrows), dim3(64), shmem, s, a, rows, out);
+      else hipLaunchKernelGGL((als_dump_kernel<NB, false>), dim3((unsigned)nrows), dim3(64), shmem, s, a, rows, out);
+    }
   }
   return hipGetLastError();
 }
@@ -481,6 +656,18 @@ hipError_t launch_rmse(const int32_t* u, const int32_t* v, const float* r, int64
 
 hipError_t launch_mfma_layout(float* out, hipStream_t s) {
   hipLaunchKernelGGL(mfma_layout_kernel, dim3(1), dim3(64), 0, s, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(const float* src, int64_t ldf, void* dst, int64_t ldb, int64_t rows, int f,
+                          hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(2048), dim3(256), 0, s, src, ldf, (__bf16*)dst, ldb, rows, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_mfma_bf16_layout(float* out, hipStream_t s) {
+  hipLaunchKernelGGL(mfma_bf16_layout_kernel, dim3(1), dim3(64), 0, s, out);
   return hipGetLastError();
 }
 

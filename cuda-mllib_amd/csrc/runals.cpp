@@ -1,0 +1,165 @@
+// runals.cpp -- standalone ALS driver (the reference's runALS.sh, without Spark).
+//
+// Reference: als/runALS.sh:18-37 takes <cpu|gpu> <master> <rank> <iterations>
+// <lambda> <file> and launches Spark's MovieLensALS, which prints the
+// held-out RMSE.  This driver takes the same knobs minus the Spark master:
+//
+//   runals gpu <rank> <iterations> <lambda> <dataset> [--split N] [--seed S]
+//
+// <dataset> is a generator preset name (see cumf_gen_preset) or
+//   csr:<m>:<n>:<nnz>:<prefix>   raw CSR files <prefix>.row.bin, .col.bin,
+//                                 .data.bin (the reference's binary format)
+// with an optional held-out COO set  --test <count>:<prefix>  (<prefix>.u.bin,
+// .v.bin int32 and .r.bin fp32).  The "cpu" mode of runALS.sh runs Spark's
+// own JVM ALS, which is not part of this library; it is rejected.
+//
+// Output: one line per iteration and a final JSON summary on stdout.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/als.h"
+#include "../../include/cumf_data.h"
+
+namespace {
+
+int usage() {
+  std::fprintf(stderr,
+               "usage: runals <cpu|gpu> <rank> <iterations> <lambda> <dataset> [--split N] [--seed S]\n"
+               "             [--test <count>:<prefix>]\n"
+               "  dataset: preset name (zipf_10k_2k_500k, netflix_480k_17770_99m, zipf_4m_500k_1b)\n"
+               "           or csr:<m>:<n>:<nnz>:<prefix> (raw <prefix>.row.bin/.col.bin/.data.bin)\n");
+  return 2;
+}
+
+bool read_file(const std::string& path, void* dst, size_t bytes) {
+  FILE* fp = std::fopen(path.c_str(), "rb");
+  if (!fp) return false;
+  const bool ok = std::fread(dst, 1, bytes, fp) == bytes && std::fgetc(fp) == EOF;
+  std::fclose(fp);
+  return ok;
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 6) return usage();
+  const std::string mode = argv[1];
+  if (mode == "cpu") {
+    std::fprintf(stderr, "runals: the cpu mode of runALS.sh runs Spark's own ALS; this driver is gpu only\n");
+    return 2;
+  }
+  if (mode != "gpu") return usage();
+  const int f = std::atoi(argv[2]);
+  const int iters = std::atoi(argv[3]);
+  const float lambda = (float)std::atof(argv[4]);
+  const std::string dataset = argv[5];
+  int split = 0;
+  uint64_t seed = 0;
+  std::string test_spec;
+  for (int i = 6; i + 1 < argc; i += 2) {
+    if (!std::strcmp(argv[i], "--split")) split = std::atoi(argv[i + 1]);
+    else if (!std::strcmp(argv[i], "--seed")) seed = std::strtoull(argv[i + 1], nullptr, 10);
+    else if (!std::strcmp(argv[i], "--test")) test_spec = argv[i + 1];
+    else return usage();
+  }
+  if (f <= 0 || iters < 0) return usage();
+
+  int64_t m = 0, n = 0, nnz = 0, ntest = 0;
+  std::vector<int32_t> rp, ci, tu, tv, cp, ri;
+  std::vector<float> rv, tr, cv;
+  const double t_load = now_s();
+  if (dataset.rfind("csr:", 0) == 0) {
+    long long mm, nn, zz;
+    char prefix[4096];
+    if (std::sscanf(dataset.c_str(), "csr:%lld:%lld:%lld:%4095s", &mm, &nn, &zz, prefix) != 4) return usage();
+    m = mm; n = nn; nnz = zz;
+    rp.resize(m + 1); ci.resize(nnz); rv.resize(nnz);
+    const std::string p = prefix;
+    if (cumf_load_csr_bin((p + ".data.bin").c_str(), (p + ".row.bin").c_str(), (p + ".col.bin").c_str(),
+                          rv.data(), rp.data(), ci.data(), (int)m, (long)nnz) != CUMF_OK) {
+      std::fprintf(stderr, "runals: %s\n", cumf_last_error());
+      return 1;
+    }
+    if (!test_spec.empty()) {
+      long long cnt;
+      char tp[4096];
+      if (std::sscanf(test_spec.c_str(), "%lld:%4095s", &cnt, tp) != 2) return usage();
+      ntest = cnt;
+      tu.resize(ntest); tv.resize(ntest); tr.resize(ntest);
+      const std::string q = tp;
+      if (!read_file(q + ".u.bin", tu.data(), 4 * ntest) || !read_file(q + ".v.bin", tv.data(), 4 * ntest) ||
+          !read_file(q + ".r.bin", tr.data(), 4 * ntest)) {
+        std::fprintf(stderr, "runals: cannot read held-out files %s.{u,v,r}.bin\n", tp);
+        return 1;
+      }
+    }
+  } else {
+    cumf_gen_params gp;
+    if (cumf_gen_preset(dataset.c_str(), &gp) != 0) return usage();
+    gp.seed = seed;
+    cumf_dataset* ds = nullptr;
+    if (cumf_gen_create(&gp, &ds) != 0) {
+      std::fprintf(stderr, "runals: generator rejected %s\n", dataset.c_str());
+      return 1;
+    }
+    cumf_gen_info_t info;
+    cumf_gen_info(ds, &info);
+    m = info.m; n = info.n; nnz = info.nnz; ntest = info.n_test;
+    rp.resize(m + 1); ci.resize(nnz); rv.resize(nnz);
+    cp.resize(n + 1); ri.resize(nnz); cv.resize(nnz);
+    tu.resize(ntest); tv.resize(ntest); tr.resize(ntest);
+    cumf_gen_copy(ds, rp.data(), ci.data(), rv.data(), cp.data(), ri.data(), cv.data(), tu.data(), tv.data(),
+                  tr.data());
+    cumf_gen_free(ds);
+  }
+  std::fprintf(stderr, "runals: %s m=%lld n=%lld nnz=%lld test=%lld loaded in %.1fs\n", dataset.c_str(),
+               (long long)m, (long long)n, (long long)nnz, (long long)ntest, now_s() - t_load);
+
+  cumf_options opt;
+  cumf_default_options(&opt);
+  opt.split = split;
+  opt.timing = 1;
+  cumf_handle* h = nullptr;
+  auto check = [&](int rc, const char* what) {
+    if (rc != CUMF_OK) {
+      std::fprintf(stderr, "runals: %s failed: %s\n", what, cumf_last_error());
+      if (h) cumf_destroy(h);
+      std::exit(1);
+    }
+  };
+  check(cumf_create(-1, f, lambda, &opt, &h), "cumf_create");
+  check(cumf_set_ratings(h, m, n, nnz, rp.data(), ci.data(), rv.data(), cp.empty() ? nullptr : cp.data(),
+                         ri.empty() ? nullptr : ri.data(), cv.empty() ? nullptr : cv.data()),
+        "cumf_set_ratings");
+  if (ntest > 0) check(cumf_set_heldout(h, ntest, tu.data(), tv.data(), tr.data()), "cumf_set_heldout");
+  std::vector<float> theta((size_t)n * f);
+  cumf_gen_init_factors(n, f, seed, theta.data());
+  check(cumf_set_factors(h, nullptr, theta.data()), "cumf_set_factors");
+
+  double total = 0.0, rmse = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    const double t0 = now_s();
+    check(cumf_iterate(h, 1), "cumf_iterate");
+    check(cumf_synchronize(h), "cumf_synchronize");
+    const double dt = now_s() - t0;
+    total += dt;
+    if (ntest > 0) check(cumf_heldout_rmse(h, &rmse), "cumf_heldout_rmse");
+    std::printf("iteration %d: %.4f s, held-out RMSE %.6f\n", it + 1, dt, rmse);
+  }
+  cumf_stats st;
+  check(cumf_get_stats(h, &st), "cumf_get_stats");
+  std::printf("{\"rank\": %d, \"iterations\": %d, \"lambda\": %g, \"m\": %lld, \"n\": %lld, \"nnz\": %lld, "
+              "\"s_per_iteration\": %.6f, \"heldout_rmse\": %.6f, \"failed_rows\": %lld}\n",
+              f, iters, (double)lambda, (long long)m, (long long)n, (long long)nnz,
+              iters ? total / iters : 0.0, rmse, (long long)st.failed_rows);
+  cumf_destroy(h);
+  return 0;
+}

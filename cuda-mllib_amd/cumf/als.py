@@ -18,11 +18,13 @@ from . import _paths
 
 CUMF_OPTIONS_VERSION = 1
 CUMF_PREC_FP32 = 0
+CUMF_PREC_BF16_GRAM = 1
 
 
 class Options(C.Structure):
     _fields_ = [("version", C.c_int32), ("precision", C.c_int32), ("split", C.c_int32),
-                ("timing", C.c_int32), ("reserved", C.c_int32 * 12)]
+                ("timing", C.c_int32), ("use_graph", C.c_int32),
+                ("compat_hermitian_bug", C.c_int32), ("reserved", C.c_int32 * 10)]
 
 
 class Stats(C.Structure):
@@ -30,7 +32,8 @@ class Stats(C.Structure):
                 ("finish_ms", C.c_double), ("finish_launches", C.c_int64),
                 ("rmse_ms", C.c_double), ("rmse_launches", C.c_int64),
                 ("sweep_ms", C.c_double), ("sweeps", C.c_int64),
-                ("failed_rows", C.c_int64), ("gram_flops", C.c_int64)]
+                ("failed_rows", C.c_int64), ("gram_flops", C.c_int64),
+                ("iter_ms", C.c_double), ("iters", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -75,6 +78,8 @@ def lib():
         "cumf_debug_gram_dim": ([C.c_int], C.c_int),
         "cumf_debug_row_gram": ([P, C.c_int, P, C.c_int64, P], C.c_int),
         "cumf_debug_mfma_layout": ([C.c_int, P], C.c_int),
+        "cumf_debug_mfma_bf16_layout": ([C.c_int, P], C.c_int),
+        "cumf_debug_compat_system": ([P, C.c_int, P, C.c_int64, P], C.c_int),
         "cumf_debug_copy": ([C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "cumf_debug_force_host_alloc": ([C.c_int], C.c_int),
         "cumf_memcpy_h2d": ([P, P, C.c_size_t], C.c_int),
@@ -82,11 +87,15 @@ def lib():
         "cumf_malloc_best": ([C.POINTER(P), C.c_size_t], C.c_int),
         "cumf_free_best": ([P], C.c_int),
         "get_gpu": ([], C.c_int),
+        "cumf_load_csr_bin": ([C.c_char_p, C.c_char_p, C.c_char_p, P, P, P, C.c_int, C.c_long], C.c_int),
+        "cumf_save_csr_bin": ([C.c_char_p, C.c_char_p, C.c_char_p, P, P, P, C.c_int, C.c_long], C.c_int),
         "doALSWithCSR": ([P, P, P, P, P, P, C.c_int, C.c_int, C.c_int, C.c_long, C.c_float, C.c_int],
                          None),
     }
     for name, (args, res) in sig.items():
-        fn = getattr(L, name)
+        fn = getattr(L, name, None)
+        if fn is None:  # entry point absent from this build: calling it raises
+            continue
         fn.argtypes = args
         fn.restype = res
     _lib = L
@@ -205,6 +214,15 @@ class ALS:
     def reset_stats(self):
         _check(lib().cumf_reset_stats(self._h), "cumf_reset_stats")
 
+    def compat_system(self, side, rows):
+        """[M | b] the compat solver sees for listed rows (f x (f+1), FP64)."""
+        rows = np.ascontiguousarray(rows, np.int32)
+        out = np.empty((rows.shape[0], self.f * (self.f + 1)), np.float64)
+        _check(lib().cumf_debug_compat_system(self._h, side, _ptr(rows), rows.shape[0], _ptr(out)),
+               "cumf_debug_compat_system")
+        M = out[:, :self.f * self.f].reshape(-1, self.f, self.f)
+        return M, out[:, self.f * self.f:]
+
     def row_gram(self, side, rows):
         """Augmented Gram (D x D, ridge included) of listed rows; side 0 = X."""
         rows = np.ascontiguousarray(rows, np.int32)
@@ -250,8 +268,10 @@ def do_als_with_csr(row_ptr, col, val, theta, m, n, f, lam, x_batch=1):
     buffers must not be freed again), like the reference's JNI bridge.
     """
     L = lib()
-    theta = np.ascontiguousarray(theta, np.float32).reshape(n, f)
-    X = np.zeros((m, f), np.float32)
+    theta = np.ascontiguousarray(theta, np.float32)
+    if f > 0:
+        theta = theta.reshape(n, f)
+    X = np.zeros((m, max(int(f), 1)), np.float32)
     bufs = [DeviceBuffer.from_array(np.asarray(row_ptr, np.int32)),
             DeviceBuffer.from_array(np.asarray(col, np.int32)),
             DeviceBuffer.from_array(np.asarray(val, np.float32))]
@@ -272,7 +292,34 @@ def mfma_layout(device=0):
     return out
 
 
+def mfma_bf16_layout(device=0):
+    out = np.empty((64, 4), np.float32)
+    _check(lib().cumf_debug_mfma_bf16_layout(device, _ptr(out)), "cumf_debug_mfma_bf16_layout")
+    return out
+
+
 def copy_kernel_ms(nbytes, iters=20, device=0):
     ms = C.c_double()
     _check(lib().cumf_debug_copy(device, int(nbytes), iters, C.byref(ms)), "cumf_debug_copy")
     return ms.value
+
+
+def save_csr_bin(prefix, row_ptr, col, val):
+    """Writes <prefix>.row.bin / .col.bin / .data.bin (raw int32/int32/fp32)."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    ci = np.ascontiguousarray(col, np.int32)
+    cv = np.ascontiguousarray(val, np.float32)
+    _check(lib().cumf_save_csr_bin(f"{prefix}.data.bin".encode(), f"{prefix}.row.bin".encode(),
+                                   f"{prefix}.col.bin".encode(), _ptr(cv), _ptr(rp), _ptr(ci),
+                                   rp.shape[0] - 1, ci.shape[0]), "cumf_save_csr_bin")
+
+
+def load_csr_bin(prefix, m, nnz):
+    """Reads the raw CSR files written by save_csr_bin (or the reference's tools)."""
+    rp = np.empty(m + 1, np.int32)
+    ci = np.empty(nnz, np.int32)
+    cv = np.empty(nnz, np.float32)
+    _check(lib().cumf_load_csr_bin(f"{prefix}.data.bin".encode(), f"{prefix}.row.bin".encode(),
+                                   f"{prefix}.col.bin".encode(), _ptr(cv), _ptr(rp), _ptr(ci), m, nnz),
+           "cumf_load_csr_bin")
+    return rp, ci, cv

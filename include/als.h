@@ -70,6 +70,9 @@ void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, f
 #define CUMF_OPTIONS_VERSION 1
 
 #define CUMF_PREC_FP32 0        /* FP32 Gram on MFMA, FP32 Cholesky */
+#define CUMF_PREC_BF16_GRAM 1   /* Gram on BF16 MFMA (FP32 accumulation) from BF16
+                                   copies of the factors, b from a BF16 hi/lo split
+                                   of the ratings, FP32 Cholesky */
 
 typedef struct cumf_options {
   int32_t version;      /* CUMF_OPTIONS_VERSION */
@@ -77,7 +80,17 @@ typedef struct cumf_options {
   int32_t split;        /* max ratings per wavefront task; longer rows are
                            split and their partial Grams reduced (0: 2048) */
   int32_t timing;       /* 1: time every kernel launch with HIP events */
-  int32_t reserved[12];
+  int32_t use_graph;    /* 1: cumf_iterate replays one iteration (X sweep +
+                           Theta sweep) captured once as a hipGraph; per-kernel
+                           times are then replaced by per-iteration times */
+  int32_t compat_hermitian_bug; /* 1: reproduce the reference's arithmetic and
+                           its Gram write slip (als/src/cuda/als.cu:275, 718):
+                           sequential FP32 Gram, element (10I+4, 10J) :=
+                           G(10I, 10J), FP64 LU without pivoting.  Needs
+                           f % 10 == 0; slow path, for parity with the
+                           reference's factors.  The legacy doALSWithCSR
+                           takes it from CUMF_COMPAT_HERMITIAN_BUG=1. */
+  int32_t reserved[10];
 } cumf_options;
 
 typedef struct cumf_stats {
@@ -91,6 +104,8 @@ typedef struct cumf_stats {
   int64_t sweeps;
   int64_t failed_rows;     /* rows with a non-positive pivot (x_u set to 0) */
   int64_t gram_flops;      /* 2 nnz f(f+1) summed over the half-sweeps run */
+  double iter_ms;          /* graph replays of whole iterations (use_graph) */
+  int64_t iters;
 } cumf_stats;
 
 typedef struct cumf_handle cumf_handle;
@@ -100,7 +115,8 @@ int cumf_default_options(cumf_options* opt);
 int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_handle** out);
 int cumf_destroy(cumf_handle* h);
 /* Host arrays; copied to the device.  The CSC (= CSR of R^T) is optional:
- * when csc_col_ptr is NULL it is built from the CSR on the host. */
+ * when csc_col_ptr is NULL it is built from the CSR on the device
+ * (cumf_csr_to_csc_device). */
 int cumf_set_ratings(cumf_handle* h, int64_t m, int64_t n, int64_t nnz, const int32_t* csr_row_ptr,
                      const int32_t* csr_col, const float* csr_val, const int32_t* csc_col_ptr,
                      const int32_t* csc_row, const float* csc_val);
@@ -134,6 +150,26 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
                            int64_t n, int f, int64_t nnz, float lambda, const cumf_options* opt);
 
 /* ------------------------------------------------------------------------
+ * Rating-matrix ingestion.
+ * ---------------------------------------------------------------------- */
+/* The reference's raw binary CSR files (loadCSRSparseMatrixBin,
+ * als/src/cuda/als.cu:914-933): int32 row_ptr[m+1], int32 col[nnz],
+ * fp32 val[nnz], no header.  Every file must exist and have exactly that
+ * size; returns CUMF_OK or CUMF_ERR_INVALID (see cumf_last_error). */
+int cumf_load_csr_bin(const char* dataFile, const char* rowFile, const char* colFile, float* data,
+                      int* row, int* col, int m, long nnz);
+int cumf_save_csr_bin(const char* dataFile, const char* rowFile, const char* colFile,
+                      const float* data, const int* row, const int* col, int m, long nnz);
+/* CSR (m x n, device arrays) -> CSC (col_ptr n+1, row nnz, val nnz, device
+ * arrays) by a stable radix sort on the device: rows stay ascending inside
+ * each column, so the result equals a host counting-sort transpose.  No
+ * counterpart in the reference (its caller transposes on the host).
+ * Synchronises `stream`. */
+int cumf_csr_to_csc_device(hipStream_t stream, int64_t m, int64_t n, int64_t nnz,
+                           const int32_t* d_row_ptr, const int32_t* d_col, const float* d_val,
+                           int32_t* d_col_ptr, int32_t* d_row, float* d_val_out);
+
+/* ------------------------------------------------------------------------
  * Test and measurement hooks (no counterpart in the reference).
  * ---------------------------------------------------------------------- */
 /* Augmented Gram of the listed rows of the X side (side 0, over the CSR)
@@ -145,9 +181,17 @@ int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t n
 /* Runs one v_mfma_f32_16x16x4_f32 with A[i][k] = 100 i + k, B[k][j] =
  * 1000 k + j and returns the 64 x 4 result registers (host out[256]). */
 int cumf_debug_mfma_layout(int device, float* out);
+/* Same for v_mfma_f32_16x16x32_bf16 with A[i][k] = i + 2 (k mod 4),
+ * B[k][j] = (k mod 5) + 3 j, k < 32, operands fed as the BF16 Gram feeds them. */
+int cumf_debug_mfma_bf16_layout(int device, float* out);
 /* Times `iters` launches of a plain copy kernel over `bytes` bytes (for
  * calibrating HBM byte counters); ms_per_launch receives the mean. */
 int cumf_debug_copy(int device, int64_t bytes, int iters, double* ms_per_launch);
+/* compat_hermitian_bug path: the system each listed row's solver sees,
+ * [M | b] as f*(f+1) doubles per row (M row-major, slip and ridge applied).
+ * side 0 = X (CSR), 1 = Theta (CSC).  out: host. */
+int cumf_debug_compat_system(cumf_handle* h, int side, const int32_t* rows, int64_t nrows,
+                             double* out);
 /* 1: mallocBest skips device memory and takes its mapped-host fallback
  * (also set by the environment variable CUMF_FORCE_HOST_ALLOC=1). */
 int cumf_debug_force_host_alloc(int on);
@@ -157,5 +201,9 @@ int cumf_memcpy_d2h(void* dst, const void* src, size_t bytes);
 
 #ifdef __cplusplus
 }
+/* Same name, C++ linkage and parameters as the reference
+ * (als/src/cuda/als.cu:914-916); status via cumf_last_error(). */
+void loadCSRSparseMatrixBin(const char* dataFile, const char* rowFile, const char* colFile,
+                            float* data, int* row, int* col, const int m, const long nnz);
 #endif
 #endif /* CUMF_ALS_H_ */

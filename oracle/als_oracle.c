@@ -38,6 +38,9 @@
  *                       lstsq on the stacked regularised system
  *   compat mode         exactly 55 changed entries at f=100 vs clean mode,
  *                       at the positions the reference writes
+ *   oracle_compat_system  FP32 fused multiply-adds in rating order, checked
+ *                       against a float64 sum (FP32-rounding bound) and the
+ *                       clean FP64 system away from the 55 slipped entries
  *   oracle_rmse         closed form on hand-made factors
  *   oracle_objective    monotone decrease across exact half-sweeps
  */
@@ -342,5 +345,40 @@ int oracle_als_train(int64_t m, int64_t n, int f, const int32_t* rp, const int32
     if (rc) return rc;
     if (rmse_hist && ho_cnt > 0) rmse_hist[it] = oracle_rmse_d(ho_cnt, ho_u, ho_v, ho_r, f, X, T, nthreads);
   }
+  return ORACLE_OK;
+}
+
+/* The system the reference's solver sees for row u, restated with the
+ * reference's own arithmetic (als/src/cuda/als.cu:111-224 for
+ * get_hermitianT10, :554-667 for get_hermitian100): each Gram entry is an
+ * FP32 register updated by one fused multiply-add per rating, in rating
+ * order; stored to FP64; the slip of :275/:718 applied; the ridge
+ * (float)(n_u * lambda) added in FP64 (:462-466, :905-909).  b is formed
+ * the same way (FP32 fused multiply-adds in rating order; the reference's
+ * cusparseScsrmm2 order is not published).
+ * M: f*f row-major doubles, b: f doubles.  Requires f % 10 == 0. */
+int oracle_compat_system(const int32_t* row_ptr, const int32_t* col, const float* val,
+                         const float* theta, int f, float lambda, int64_t u, double* M, double* b) {
+  if (f <= 0 || f % 10 != 0) return ORACLE_EINVAL;
+  float* G = (float*)calloc((size_t)f * f, sizeof(float));
+  float* bf = (float*)calloc((size_t)f, sizeof(float));
+  if (!G || !bf) { free(G); free(bf); return ORACLE_ENOMEM; }
+  const int64_t s = row_ptr[u], e = row_ptr[u + 1];
+  for (int64_t k = s; k < e; ++k) {
+    const float* t = theta + (int64_t)col[k] * f;
+    for (int i = 0; i < f; ++i) {
+      for (int j = 0; j < f; ++j) G[(size_t)i * f + j] = fmaf(t[i], t[j], G[(size_t)i * f + j]);
+      bf[i] = fmaf(val[k], t[i], bf[i]);
+    }
+  }
+  for (size_t i = 0; i < (size_t)f * f; ++i) M[i] = (double)G[i];
+  apply_compat_slip(f, M);
+  const double reg = (double)((float)(e - s) * lambda);
+  for (int i = 0; i < f; ++i) {
+    M[(size_t)i * f + i] += reg;
+    b[i] = (double)bf[i];
+  }
+  free(G);
+  free(bf);
   return ORACLE_OK;
 }

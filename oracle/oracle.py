@@ -23,6 +23,7 @@ def lib():
         L.oracle_max_threads.restype = C.c_int
         L.oracle_row_system.argtypes = [P, P, P, P, I64, C.c_int, C.c_float, I64, C.c_int, P, P]
         L.oracle_chol_solve.argtypes = [C.c_int, P, P]
+        L.oracle_compat_system.argtypes = [P, P, P, P, C.c_int, C.c_float, I64, P, P]
         L.oracle_lu_solve.argtypes = [C.c_int, P, P]
         L.oracle_half_sweep_d.argtypes = [I64, I64, C.c_int, P, P, P, P, C.c_float, C.c_int, P, P, C.c_int]
         L.oracle_half_sweep_f.argtypes = [I64, I64, C.c_int, P, P, P, P, C.c_float, C.c_int, P, P, C.c_int]
@@ -62,6 +63,19 @@ def row_system(row_ptr, col, val, theta, f, lam, u, compat=False):
     if rc:
         raise ValueError(f"oracle_row_system: {rc}")
     return A, b
+
+
+def compat_system(row_ptr, col, val, theta, f, lam, u):
+    """(M, b) the reference's solver sees for row u: FP32 fused multiply-adds
+    in rating order, the Gram write slip, the FP32 ridge (FP64 out)."""
+    row_ptr, col, val = _c(row_ptr, np.int32), _c(col, np.int32), _c(val, np.float32)
+    theta = _c(theta, np.float32)
+    M = np.empty((f, f), np.float64)
+    b = np.empty(f, np.float64)
+    rc = lib().oracle_compat_system(_p(row_ptr), _p(col), _p(val), _p(theta), f, lam, int(u), _p(M), _p(b))
+    if rc:
+        raise ValueError(f"oracle_compat_system: {rc}")
+    return M, b
 
 
 def chol_solve(A, b):

@@ -33,8 +33,10 @@ def test_gpuals_exports_every_declared_entry_point():
             | {"Java_org_apache_spark_ml_recommendation_CuMFJNIInterface_doALSWithCSR",
                "Java_org_apache_spark_ml_recommendation_CuMFJNIInterface_testjni",
                "cumf_jni_flatten_factors", "cumf_jni_rows_to_array"})
-    # C++-linkage allocation contract of the reference (utilities.h:36-41)
-    want |= {"_Z10mallocBestPPvm", "_Z8freeBestPv"}
+    # C++-linkage symbols of the reference: allocation contract
+    # (utilities.h:36-41) and the binary CSR loader (als/src/cuda/als.cu:914)
+    want -= {"loadCSRSparseMatrixBin"}
+    want |= {"_Z10mallocBestPPvm", "_Z8freeBestPv", "_Z22loadCSRSparseMatrixBinPKcS0_S0_PfPiS2_il"}
     missing = sorted(want - exp)
     assert not missing, missing
     assert "doALSWithCSR" in exp

@@ -204,3 +204,33 @@ def test_objective_closed_form():
     # loss: (3-2)^2 + (1-2)^2 = 2; reg: lam*(1*1 + 1*4 + 1*4 + 1*1) = 10 lam
     lam = 0.5
     assert abs(O.objective(rp, ci, cv, X, T, lam) - (2 + 10 * lam)) < 1e-12
+
+
+@pytest.mark.parametrize("f", [10, 100])
+def test_compat_system_reference_arithmetic(f):
+    """The FP32 sequential-FMA system agrees with the FP64 clean system to
+    FP32 rounding away from the slipped entries, carries the slip exactly,
+    and solving it by LU reproduces the golden compat solution closely."""
+    theta = FIX[f"theta_f{f}"]
+    rp, ci, cv = FIX["rp"], FIX["ci"], FIX["cv"]
+    for u in (0, 1, 2):
+        M, b = O.compat_system(rp, ci, cv, theta, f, LAM, u)
+        Ac, bc = O.row_system(rp, ci, cv, theta, f, LAM, u, compat=True)
+        n_u = rp[u + 1] - rp[u]
+        tol = 4 * n_u * np.finfo(np.float32).eps * (np.abs(Ac).max() + 1)
+        assert np.abs(M - Ac).max() <= tol
+        assert np.abs(b - bc).max() <= 4 * n_u * np.finfo(np.float32).eps * (np.abs(bc).max() + 5)
+        ridge = float(np.float32(np.float32(n_u) * np.float32(LAM)))
+        for I in range(f // 10):
+            for J in range(I, f // 10):
+                src = M[10 * I, 10 * J] - (ridge if I == J else 0.0)
+                assert abs(M[10 * I + 4, 10 * J] - src) <= (1e-12 * abs(src) if I == J else 0.0)
+        # sequential fmaf: entries of symmetric positions agree bit for bit
+        G = M - np.diag(np.diag(M))
+        mask = np.ones((f, f), bool)
+        for I in range(f // 10):
+            for J in range(I, f // 10):
+                mask[10 * I + 4, 10 * J] = mask[10 * J, 10 * I + 4] = False
+        np.testing.assert_array_equal(G[mask], G.T[mask])
+    x = O.lu_solve(*O.compat_system(rp, ci, cv, theta, f, LAM, 0))
+    np.testing.assert_allclose(x, FIX[f"X1compat_f{f}"][0], rtol=1e-4, atol=1e-5)
