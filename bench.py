@@ -84,6 +84,10 @@ def main():
     ap.add_argument("--lam", type=float, default=0.058)
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "bf16gram"], default="fp32")
+    ap.add_argument("--solver", choices=["cholesky", "cg"], default="cholesky")
+    ap.add_argument("--cg-iters", type=int, default=0)
+    ap.add_argument("--cg-rounds", type=int, default=0)
+    ap.add_argument("--cg-tol", type=float, default=0.0)
     ap.add_argument("--graph", action="store_true", help="replay each iteration as a captured hipGraph")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -111,7 +115,9 @@ def main():
     f, lam = args.f, args.lam
     theta0 = cumf.init_factors(ds.n, f, 0)
     prec = {"fp32": G.CUMF_PREC_FP32, "bf16gram": G.CUMF_PREC_BF16_GRAM}[args.precision]
-    h = G.ALS(f, lam, device=0, timing=1, split=args.split, precision=prec, use_graph=int(args.graph))
+    solver = {"cholesky": G.CUMF_SOLVER_CHOLESKY, "cg": G.CUMF_SOLVER_CG}[args.solver]
+    h = G.ALS(f, lam, device=0, timing=1, split=args.split, precision=prec, use_graph=int(args.graph),
+              solver=solver, cg_iters=args.cg_iters, cg_rounds=args.cg_rounds, cg_tol=args.cg_tol)
     h.set_dataset(ds)
     h.set_factors(theta=theta0)
     log("ratings and factors resident on device")
@@ -128,6 +134,15 @@ def main():
     rmse = h.rmse()
     X, T = h.get_factors()
     log(f"timed {args.steps} iterations: {elapsed:.3f}s; rmse={rmse:.5f}; stats={st}")
+    # Per-half-sweep split, after the timed region (one more iteration).
+    sweep_ms = {}
+    for name, fn in (("x", h.update_x), ("theta", h.update_theta)):
+        h.synchronize()
+        t1 = time.perf_counter()
+        fn()
+        h.synchronize()
+        sweep_ms[name] = (time.perf_counter() - t1) * 1e3
+    opt = h.opt
     h.close()
 
     s_per_iter = elapsed / args.steps
@@ -151,17 +166,23 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if args.precision == "fp32" else "bf16 gram / fp32 solve",
+        "dtype": ("fp32" if args.precision == "fp32" else "bf16 gram")
+                 + (" / fp32 solve" if args.solver == "cholesky" else " / fp16 cg + fp32 refinement + fp64 residual"),
         "data": "synthetic",
         "config": {"workload": args.workload, "m": ds.m, "n": ds.n, "nnz": ds.nnz, "f": f,
                    "lambda": lam, "precision": {"fp32": "fp32 gram (MFMA) + fp32 cholesky",
                                                 "bf16gram": "bf16 gram (MFMA, fp32 accumulate) + fp32 cholesky"}[args.precision],
+                   "solver": args.solver if args.solver == "cholesky" else
+                   f"cg (fp16, {opt.cg_iters} steps x {opt.cg_rounds} fp32 refinement rounds, fp64 residual, "
+                   f"tol {opt.cg_tol:g}, cholesky fallback)",
                    "graph": bool(args.graph),
                    "split": args.split or 2048},
         "ratings_per_s": ds.nnz / s_per_iter,
         "hermitian_tflops": gram_flops_iter / (kern_ms_total / args.steps * 1e-3) / 1e12,
         "heldout_rmse": rmse,
         "failed_rows": st["failed_rows"],
+        "cg_fallback_rows_per_iter": st["cg_fallback_rows"] / args.steps,
+        "half_sweep_ms": sweep_ms,
         "kernel_ms_per_iter": {"als_task_kernel": st["task_ms"] / args.steps,
                                "als_finish_kernel": st["finish_ms"] / args.steps},
         "roofline": {"kernel": kname, "bound": "mfma",

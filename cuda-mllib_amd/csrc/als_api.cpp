@@ -29,6 +29,9 @@ namespace {
 
 constexpr int kMaxF = 140;
 constexpr int kDefaultSplit = 2048;
+constexpr int kDefaultCgIters = 6;
+constexpr int kDefaultCgRounds = 2;
+constexpr float kDefaultCgTol = 1e-3f;
 
 int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }
 
@@ -254,6 +257,11 @@ int run_half_sweep(cumf_handle* h, const Plan& p, const DevBuf& rowp, const DevB
   a.theta_b = theta_b.as<uint16_t>();
   a.x_b = x_b.as<uint16_t>();
   a.zero16 = h->zero16.p;
+  a.solver = h->opt.solver;
+  a.cg_iters = h->opt.cg_iters;
+  a.cg_rounds = h->opt.cg_rounds;
+  a.cg_tol = h->opt.cg_tol;
+  a.cg_fallback = h->fail.as<int>() + 1;
   hipEvent_t s0 = nullptr, s1 = nullptr;
   if (h->opt.timing && !h->capturing) {
     s0 = h->timer.get();
@@ -291,6 +299,10 @@ int cumf_default_options(cumf_options* opt) {
   opt->precision = CUMF_PREC_FP32;
   opt->split = kDefaultSplit;
   opt->timing = 0;
+  opt->solver = CUMF_SOLVER_CHOLESKY;
+  opt->cg_iters = kDefaultCgIters;
+  opt->cg_rounds = kDefaultCgRounds;
+  opt->cg_tol = kDefaultCgTol;
   return CUMF_OK;
 }
 
@@ -312,6 +324,21 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
     }
     o = *opt;
     if (o.split <= 0) o.split = kDefaultSplit;
+  }
+  if (o.cg_iters <= 0) o.cg_iters = kDefaultCgIters;
+  if (o.cg_rounds <= 0) o.cg_rounds = kDefaultCgRounds;
+  if (!(o.cg_tol > 0.f)) o.cg_tol = kDefaultCgTol;
+  if (o.solver != CUMF_SOLVER_CHOLESKY && o.solver != CUMF_SOLVER_CG) {
+    cumf::set_last_error("solver not built");
+    return CUMF_ERR_UNSUPPORTED;
+  }
+  if (o.solver != CUMF_SOLVER_CHOLESKY && o.compat_hermitian_bug) {
+    cumf::set_last_error("compat_hermitian_bug reproduces the reference's LU solve; use CUMF_SOLVER_CHOLESKY");
+    return CUMF_ERR_INVALID;
+  }
+  if (o.cg_iters > 64 || o.cg_rounds > 16) {
+    cumf::set_last_error("cg_iters must be <= 64 and cg_rounds <= 16");
+    return CUMF_ERR_INVALID;
   }
   if (o.precision != CUMF_PREC_FP32 && o.precision != CUMF_PREC_BF16_GRAM) {
     cumf::set_last_error("precision mode not built");
@@ -351,12 +378,12 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
     cumf::set_last_error("hipStreamCreate failed");
     return CUMF_ERR_HIP;
   }
-  int rc = h->fail.alloc(sizeof(int));
+  int rc = h->fail.alloc(2 * sizeof(int));  // failed rows, CG fallbacks
   if (rc == CUMF_OK) rc = h->rmse_partial.alloc(sizeof(double) * kRmseBlocks);
   if (rc == CUMF_OK) rc = h->rmse_out.alloc(sizeof(double));
   if (rc == CUMF_OK) rc = h->zero16.alloc(64);
   if (rc == CUMF_OK && hipMemset(h->zero16.p, 0, 64) != hipSuccess) rc = CUMF_ERR_HIP;
-  if (rc == CUMF_OK && hipMemset(h->fail.p, 0, sizeof(int)) != hipSuccess) rc = CUMF_ERR_HIP;
+  if (rc == CUMF_OK && hipMemset(h->fail.p, 0, 2 * sizeof(int)) != hipSuccess) rc = CUMF_ERR_HIP;
   if (rc != CUMF_OK) {
     (void)hipStreamDestroy(h->stream);
     delete h;
@@ -630,8 +657,8 @@ int cumf_get_stats(cumf_handle* h, cumf_stats* s) {
   if (!h || !s) return CUMF_ERR_INVALID;
   CUMF_TRY(cumf_synchronize(h));
   DeviceGuard dg(h->device);
-  int fails = 0;
-  CUMF_HIP_TRY(hipMemcpy(&fails, h->fail.p, sizeof(int), hipMemcpyDeviceToHost));
+  int fails[2] = {0, 0};
+  CUMF_HIP_TRY(hipMemcpy(fails, h->fail.p, 2 * sizeof(int), hipMemcpyDeviceToHost));
   std::memset(s, 0, sizeof(*s));
   s->task_ms = h->timer.ms[Timer::TASK];
   s->task_launches = h->timer.count[Timer::TASK];
@@ -641,7 +668,8 @@ int cumf_get_stats(cumf_handle* h, cumf_stats* s) {
   s->rmse_launches = h->timer.count[Timer::RMSE];
   s->sweep_ms = h->timer.ms[Timer::SWEEP];
   s->sweeps = h->timer.count[Timer::SWEEP];
-  s->failed_rows = fails;
+  s->failed_rows = fails[0];
+  s->cg_fallback_rows = fails[1];
   s->gram_flops = h->gram_flops;
   s->iter_ms = h->timer.ms[Timer::ITER];
   s->iters = h->timer.count[Timer::ITER];
@@ -657,7 +685,7 @@ int cumf_reset_stats(cumf_handle* h) {
     h->timer.count[k] = 0;
   }
   h->gram_flops = 0;
-  CUMF_HIP_TRY(hipMemset(h->fail.p, 0, sizeof(int)));
+  CUMF_HIP_TRY(hipMemset(h->fail.p, 0, 2 * sizeof(int)));
   return CUMF_OK;
 }
 

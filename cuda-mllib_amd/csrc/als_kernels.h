@@ -47,6 +47,14 @@ struct SweepArgs {
   const uint16_t* theta_b;
   uint16_t* x_b;
   const void* zero16;      // 16 zero bytes (gather source for padding)
+  // solver != 0: FP16 conjugate gradients inside FP32 refinement with an
+  // FP64 residual, warm-started from x; Cholesky for rows that do not reach
+  // ||b - A x|| <= cg_tol ||b|| within cg_rounds rounds of cg_iters steps.
+  int solver;
+  int cg_iters;
+  int cg_rounds;
+  float cg_tol;
+  int* cg_fallback;        // rows that fell back to Cholesky
 };
 
 // The reference-reproducing path (compat.hip): one work-group per row.

@@ -56,7 +56,7 @@ __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ constexpr int lds_floats_nb(int NB, int f) {
   return cmax(cmax(2 * KB * ((f + 3) & ~3) + 2 * KB,   // FP32 staging
                    2 * 32 * 16 * NB / 2),              // BF16 staging (2 x 32 rows x 16NB bf16)
-              cmax(f * (f + 1) / 2 + 2 * f, 16 * NB));  // U, z, z2; pivot-row buffer
+              cmax(f * (f + 1) / 2 + 2 * f, 32 * NB));  // U, z, z2; two pivot-row buffers
 }
 
 
@@ -254,69 +254,106 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
   wait_vm_all();
 }
 
-// Factorisation steps k in [16K, 16K+16) of block row K, then the next
-// block row.  K and the register index kr are compile-time so every
-// accumulator access is a fixed register; the 16-lane group kg that owns
-// row k is a run-time loop.
-template <int NB, int K>
-__device__ __forceinline__ void chol_block(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ rowbuf,
-                                           bool& ok, int lane) {
+// Right-looking Cholesky on the accumulator layout, software-pipelined by
+// one pivot.  Step k uses the pivot row a_k* (its diagonal d and the values
+// colv[J] = a_k,16J+c were read during step k-1), then:
+//   1. computes row k+1's updated values first (temporaries) and has its
+//      16 owner lanes broadcast them through the other half of a two-row
+//      LDS buffer, and issues the reads of the next pivot and of colv;
+//   2. applies the rank-1 update to every element it owns (the same fused
+//      multiply-adds, so row k+1 ends bit-identical to its broadcast copy).
+// The LDS round trip of pivot k+1 thus overlaps the bulk update of pivot k.
+// K (block row) and KR (register of the pivot row) are compile-time, so
+// every accumulator access is a fixed register; the 16-lane group kg that
+// owns row k is a run-time loop.
+
+// LASTG: kg == 3 is known at compile time, so the pivot after KR == 3 is
+// the first row of block K+1 (a fixed register) rather than of group kg+1.
+// The pivot row's values a_k,16I+4g+{0..3} are streamed block by block
+// (one block in use, one prefetched) to keep the register footprint small.
+template <int NB, int K, int KR, bool LASTG>
+__device__ __forceinline__ void chol_step(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ rb, bool& ok,
+                                          int lane, int kg, float& d, float (&colv)[NB]) {
   const int g = lane >> 4, c = lane & 15;
-  if (16 * K < f) {
-#pragma unroll 1
-    for (int kg = 0; kg < 4; ++kg) {
+  const int k = 16 * K + 4 * kg + KR;
+  if (k >= f) return;  // uniform
+  const float* cur = rb + (k & 1) * 16 * NB;
+  float* nxt = rb + ((k + 1) & 1) * 16 * NB;
+  f4 rvK = *reinterpret_cast<const f4*>(cur + 16 * K + 4 * g);
+  f4 rvN = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (K + 1 < NB) rvN = *reinterpret_cast<const f4*>(cur + 16 * (K + 1) + 4 * g);
+  const bool good = d > 0.0f && d < __builtin_huge_valf();
+  ok = ok && good;
+  const float inv = good ? __builtin_amdgcn_rsqf(d) : 0.0f;  // 1/sqrt(d)
+  const float inv2 = inv * inv;                               // 1/d
+  float cj[NB];
 #pragma unroll
-      for (int kr = 0; kr < 4; ++kr) {
-        const int k = 16 * K + 4 * kg + kr;
-        if (k < f) {
-          // Row k of the trailing matrix, a_k*, from its 16 owner lanes.
-          if (g == kg) {
+  for (int J = K; J < NB; ++J) cj[J] = colv[J] * inv2;
+  // Rows of block K at or above k are finished; later blocks are all below k.
 #pragma unroll
-            for (int J = K; J < NB; ++J) rowbuf[16 * J + c] = acc[tile_index(K, J, NB)][kr];
-          }
-          lds_order();
-          const float d = rowbuf[k];
-          const bool good = d > 0.0f && d < __builtin_huge_valf();
-          ok = ok && good;
-          const float inv = good ? __builtin_amdgcn_rsqf(d) : 0.0f;  // 1/sqrt(d)
-          const float inv2 = inv * inv;                               // 1/d
-          // a_ij -= a_ki a_kj / d  for rows i > k; u_kj = a_kj / sqrt(d).
-          float cj[NB], uj[NB];
+  for (int r = 0; r < 4; ++r) rvK[r] = (4 * g + r > 4 * kg + KR) ? rvK[r] : 0.0f;
+  if (g == kg) {  // row k of U
 #pragma unroll
-          for (int J = K; J < NB; ++J) {
-            const float v = rowbuf[16 * J + c];
-            cj[J] = v * inv2;
-            uj[J] = v * inv;
-          }
-          f4 rv[NB];
+    for (int J = K; J < NB; ++J) acc[tile_index(K, J, NB)][KR] = colv[J] * inv;
+  }
+  if (k + 1 < f) {
+    // Row k+1's owners: same block and group (KR < 3), the next group, or
+    // group 0 of the next block.
+    constexpr int R1 = KR < 3 ? KR + 1 : 0;
+    if constexpr (KR < 3 || !LASTG) {
+      const int g1 = KR < 3 ? kg : kg + 1;
+      if (g == g1) {
 #pragma unroll
-          for (int I = K; I < NB; ++I) rv[I] = *reinterpret_cast<const f4*>(rowbuf + 16 * I + 4 * g);
-          // Rows of block K at or above k are finished; later blocks are all below k.
+        for (int J = K; J < NB; ++J) nxt[16 * J + c] = fmaf(-rvK[R1], cj[J], acc[tile_index(K, J, NB)][R1]);
+      }
+    } else if constexpr (K + 1 < NB) {
+      if (g == 0) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) rv[K][r] = (4 * g + r > 4 * kg + kr) ? rv[K][r] : 0.0f;
-          lds_order();
-          if (g == kg) {
-#pragma unroll
-            for (int J = K; J < NB; ++J) acc[tile_index(K, J, NB)][kr] = uj[J];
-          }
-#pragma unroll
-          for (int I = K; I < NB; ++I)
-#pragma unroll
-            for (int J = I; J < NB; ++J)
-#pragma unroll
-              for (int r = 0; r < 4; ++r)
-                acc[tile_index(I, J, NB)][r] = fmaf(-rv[I][r], cj[J], acc[tile_index(I, J, NB)][r]);
-        }
+        for (int J = K + 1; J < NB; ++J)
+          nxt[16 * J + c] = fmaf(-rvN[0], cj[J], acc[tile_index(K + 1, J, NB)][0]);
       }
     }
+    lds_order();
+    d = nxt[k + 1];
+#pragma unroll
+    for (int J = K; J < NB; ++J) colv[J] = nxt[16 * J + c];
   }
-  if constexpr (K + 1 < NB) chol_block<NB, K + 1>(acc, f, rowbuf, ok, lane);
+  // Rank-1 update of every element owned, block row by block row.
+  f4 rv = rvK;  // block I
+  f4 nx = rvN;  // block I+1
+#pragma unroll
+  for (int I = K; I < NB; ++I) {
+    f4 nn = {0.f, 0.f, 0.f, 0.f};  // block I+2
+    if (I + 2 < NB) nn = *reinterpret_cast<const f4*>(cur + 16 * (I + 2) + 4 * g);
+#pragma unroll
+    for (int J = I; J < NB; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        acc[tile_index(I, J, NB)][r] = fmaf(-rv[r], cj[J], acc[tile_index(I, J, NB)][r]);
+    rv = nx;
+    nx = nn;
+  }
 }
 
-// ---------------------------------------------------------------------------
-// In-register Cholesky of the augmented Gram + back substitution.
-// Returns false if a pivot is not positive (x is then written as zeros).
-// ---------------------------------------------------------------------------
+template <int NB, int K>
+__device__ __forceinline__ void chol_block(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ rb, bool& ok,
+                                           int lane, float& d, float (&colv)[NB]) {
+  if (16 * K < f) {
+#pragma unroll 1
+    for (int kg = 0; kg < 3; ++kg) {
+      chol_step<NB, K, 0, false>(acc, f, rb, ok, lane, kg, d, colv);
+      chol_step<NB, K, 1, false>(acc, f, rb, ok, lane, kg, d, colv);
+      chol_step<NB, K, 2, false>(acc, f, rb, ok, lane, kg, d, colv);
+      chol_step<NB, K, 3, false>(acc, f, rb, ok, lane, kg, d, colv);
+    }
+    chol_step<NB, K, 0, true>(acc, f, rb, ok, lane, 3, d, colv);
+    chol_step<NB, K, 1, true>(acc, f, rb, ok, lane, 3, d, colv);
+    chol_step<NB, K, 2, true>(acc, f, rb, ok, lane, 3, d, colv);
+    chol_step<NB, K, 3, true>(acc, f, rb, ok, lane, 3, d, colv);
+  }
+  if constexpr (K + 1 < NB) chol_block<NB, K + 1>(acc, f, rb, ok, lane, d, colv);
+}
+
 // bcol2 >= 0: a second right-hand-side column whose solution is added to
 // the first (the BF16 low part of the ratings).  xb: optional BF16 copy of
 // the solution (row of ldb entries, zero padded).
@@ -327,7 +364,19 @@ __device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int 
                                            __bf16* __restrict__ xb = nullptr, int64_t ldb = 0) {
   const int g = lane >> 4, c = lane & 15;
   bool ok = true;
-  chol_block<NB, 0>(acc, f, lds, ok, lane);
+  {
+    // Pivot row 0 (group 0, register 0 of tiles (0, J)) into the first buffer.
+    if (g == 0) {
+#pragma unroll
+      for (int J = 0; J < NB; ++J) lds[16 * J + c] = acc[tile_index(0, J, NB)][0];
+    }
+    lds_order();
+    float d = lds[0];
+    float colv[NB];
+#pragma unroll
+    for (int J = 0; J < NB; ++J) colv[J] = lds[16 * J + c];
+    chol_block<NB, 0>(acc, f, lds, ok, lane, d, colv);
+  }
 
   // Spill U (packed by columns: U[i][j] at j(j+1)/2 + i) and z.
   const int upk = f * (f + 1) / 2;
@@ -393,6 +442,338 @@ __device__ __forceinline__ void add_ridge(f4 (&acc)[num_tiles(NB)], int f, float
 }
 
 // ---------------------------------------------------------------------------
+// Mixed-precision solve: conjugate gradients on the Jacobi-scaled matrix
+// A^ = D^-1/2 A D^-1/2 held in LDS as FP16 (unit diagonal, |entries| <= 1,
+// so FP16 holds it without range trouble), inside FP32 iterative
+// refinement whose residual r = b - A x is formed in FP64 from the FP32
+// Gram still in registers.  Each refinement round solves A^ d = D^-1/2 r /
+// mu (mu = max |D^-1/2 r|, so the FP16 search directions stay O(1)) with a
+// fixed number of CG iterations from d = 0 and adds mu D^-1/2 d to x.  The
+// start is the row's current factor (the previous sweep's value).  A row
+// whose relative residual ||b - A x|| / ||b|| has not reached the tolerance
+// after the last round is solved again by the Cholesky path.
+//
+// LDS (after the Gram phase): A^ (f rows, stride cg_stride(f) halves), the
+// FP16 direction p, x (16NB floats) and y = A x (16NB doubles).  Vectors
+// live one entry per lane (entry i in lane i % 64, slot i / 64).
+// ---------------------------------------------------------------------------
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+// Row stride of A^ in halves: f rounded up to 8, then to an odd multiple of
+// 8, so that 16-byte row reads of consecutive lanes hit distinct banks.
+__host__ __device__ constexpr int cg_stride(int f) {
+  return (((f + 7) / 8) & 1) ? ((f + 7) & ~7) : ((f + 7) & ~7) + 8;
+}
+__host__ __device__ constexpr int cg_lds_floats(int NB, int f) {
+  return (f + 1) * cg_stride(f) / 2 + 16 * NB + 32 * NB;
+}
+__host__ __device__ constexpr int rows_per_lane(int NB) { return (16 * NB + 63) / 64; }
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// Sum / max over the wave, returned uniform: quad butterflies and two row
+// rotations give every lane its 16-lane row total; the four row totals are
+// combined in a fixed order from lanes 0, 16, 32, 48.
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x124>(v);  // row_ror:4
+  v += dpp_f<0x128>(v);  // row_ror:8
+  return (readlane_f(v, 0) + readlane_f(v, 16)) + (readlane_f(v, 32) + readlane_f(v, 48));
+}
+__device__ __forceinline__ float wave_max(float v) {
+  v = fmaxf(v, dpp_f<0xB1>(v));
+  v = fmaxf(v, dpp_f<0x4E>(v));
+  v = fmaxf(v, dpp_f<0x124>(v));
+  v = fmaxf(v, dpp_f<0x128>(v));
+  return fmaxf(fmaxf(readlane_f(v, 0), readlane_f(v, 16)), fmaxf(readlane_f(v, 32), readlane_f(v, 48)));
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return readlane_d(v, 0);
+}
+
+// ys[0, 16NB) = A xs in FP64, A the Gram in registers (upper tiles; the
+// diagonal tiles are full), xs zero beyond f.  Row direction: lane (g, c)
+// of tile (I, J) adds A[16I+4g+r][16J+c] x[16J+c] to row 16I+4g+r; the 16
+// lanes of a group are reduced with a reduce-scatter that leaves row
+// 16I+4g+(c>>2) in lanes c with equal c>>2.  Column direction (strictly
+// upper tiles, the lower triangle by symmetry): row 16J+c collects
+// A[16I+4g+r][16J+c] x[16I+4g+r] over r, I < J and the four groups.
+template <int NB>
+__device__ __forceinline__ void gram_matvec_f64(const f4 (&acc)[num_tiles(NB)], const float* __restrict__ xs,
+                                                double* __restrict__ ys, int lane) {
+  const int g = lane >> 4, c = lane & 15;
+  double xc[NB];
+#pragma unroll
+  for (int J = 0; J < NB; ++J) xc[J] = (double)xs[16 * J + c];
+  const bool h8 = (c & 8) != 0, h4b = (c & 4) != 0;
+#pragma unroll
+  for (int I = 0; I < NB; ++I) {
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+#pragma unroll
+    for (int J = I; J < NB; ++J) {
+      const f4 v = acc[tile_index(I, J, NB)];
+      y0 = fma((double)v[0], xc[J], y0);
+      y1 = fma((double)v[1], xc[J], y1);
+      y2 = fma((double)v[2], xc[J], y2);
+      y3 = fma((double)v[3], xc[J], y3);
+    }
+    double k0 = h8 ? y2 : y0, k1 = h8 ? y3 : y1;
+    const double s0 = h8 ? y0 : y2, s1 = h8 ? y1 : y3;
+    k0 += __shfl_xor(s0, 8, 64);
+    k1 += __shfl_xor(s1, 8, 64);
+    double k = h4b ? k1 : k0;
+    const double s = h4b ? k0 : k1;
+    k += __shfl_xor(s, 4, 64);
+    k += __shfl_xor(k, 2, 64);
+    k += __shfl_xor(k, 1, 64);
+    if ((c & 3) == 0) ys[16 * I + 4 * g + (c >> 2)] = k;
+  }
+  lds_order();
+#pragma unroll
+  for (int J = 1; J < NB; ++J) {
+    double yc = 0.0;
+#pragma unroll
+    for (int I = 0; I < J; ++I) {
+      const f4 xr = *reinterpret_cast<const f4*>(xs + 16 * I + 4 * g);
+      const f4 v = acc[tile_index(I, J, NB)];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) yc = fma((double)v[r], (double)xr[r], yc);
+    }
+    yc += __shfl_xor(yc, 16, 64);
+    yc += __shfl_xor(yc, 32, 64);
+    if (g == 0) ys[16 * J + c] += yc;
+  }
+  lds_order();
+}
+
+// q = A^ p for the rows this lane owns (FP16 operands, FP32 accumulation).
+// Rows past f are computed on row f-1 (same-address reads broadcast) and
+// zeroed.
+template <int NB>
+__device__ __forceinline__ void cg_matvec(const _Float16* __restrict__ Ah, const _Float16* __restrict__ p16,
+                                          int f, int sa, int nch, int lane, float (&q)[rows_per_lane(NB)]) {
+  constexpr int RPL = rows_per_lane(NB);
+  const _Float16* rowp[RPL];
+  float q2[RPL];
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int i = lane + 64 * t;
+    rowp[t] = Ah + (i < f ? i : f - 1) * sa;
+    q[t] = 0.0f;
+    q2[t] = 0.0f;
+  }
+#pragma unroll 2
+  for (int k = 0; k < nch; ++k) {
+    const h8 pk = *reinterpret_cast<const h8*>(p16 + 8 * k);
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const h8 av = *reinterpret_cast<const h8*>(rowp[t] + 8 * k);
+      q[t] = __builtin_amdgcn_fdot2(h2{av[0], av[1]}, h2{pk[0], pk[1]}, q[t], false);
+      q2[t] = __builtin_amdgcn_fdot2(h2{av[2], av[3]}, h2{pk[2], pk[3]}, q2[t], false);
+      q[t] = __builtin_amdgcn_fdot2(h2{av[4], av[5]}, h2{pk[4], pk[5]}, q[t], false);
+      q2[t] = __builtin_amdgcn_fdot2(h2{av[6], av[7]}, h2{pk[6], pk[7]}, q2[t], false);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) q[t] = (lane + 64 * t < f) ? q[t] + q2[t] : 0.0f;
+}
+
+struct CgParams {
+  int iters;   // CG iterations per refinement round
+  int rounds;  // refinement rounds
+  float tol;   // accept x once ||b - A x|| <= tol ||b||
+};
+
+// Returns 1 if x was accepted, 0 if the Cholesky fallback ran (ok reports
+// the fallback's pivots), and writes x to xrow / xb as chol_solve does.
+template <int NB>
+__device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bcol, float* __restrict__ lds,
+                                         float* __restrict__ xrow, int64_t ldf, int lane, int bcol2,
+                                         __bf16* __restrict__ xb, int64_t ldb, const CgParams& cp, bool& ok) {
+  constexpr int RPL = rows_per_lane(NB);
+  constexpr int D = 16 * NB;
+  const int g = lane >> 4, c = lane & 15;
+  const int sa = cg_stride(f);
+  const int f8 = (f + 7) & ~7;
+  _Float16* Ah = reinterpret_cast<_Float16*>(lds);
+  _Float16* p16 = Ah + f * sa;
+  float* xs = lds + (f + 1) * sa / 2;
+  double* ys = reinterpret_cast<double*>(xs + D);
+  float* ds = xs;                                  // diagonal, during set-up
+  float* bs = reinterpret_cast<float*>(ys);        // b (and the BF16 low part at bs + D)
+  float* ss = reinterpret_cast<float*>(ys);        // Jacobi scales, after b is read
+
+  // Diagonal and right-hand side out of the register tiles.
+  lds_order();
+#pragma unroll
+  for (int I = 0; I < NB; ++I)
+#pragma unroll
+    for (int J = I; J < NB; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * I + 4 * g + r, j = 16 * J + c;
+        const float v = acc[tile_index(I, J, NB)][r];
+        if (I == J && i == j && i < f) ds[i] = v;
+        if (j == bcol && i < f) bs[i] = v;
+        if (j == bcol2 && i < f) bs[D + i] = v;
+      }
+  lds_order();
+  float sc[RPL], bv[RPL], xv[RPL];
+  bool bad = false;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int i = lane + 64 * t;
+    const bool in = i < f;
+    const float d = in ? ds[i] : 1.0f;
+    const bool good = d > 0.0f && d < __builtin_huge_valf();
+    bad = bad || !good;
+    sc[t] = (in && good) ? __builtin_amdgcn_rsqf(d) : 0.0f;
+    bv[t] = in ? bs[i] + (bcol2 >= 0 ? bs[D + i] : 0.0f) : 0.0f;
+    float x0 = in ? xrow[i] : 0.0f;
+    if (!(fabsf(x0) < __builtin_huge_valf())) x0 = 0.0f;
+    xv[t] = x0;
+  }
+  const bool any_bad = __any(bad);
+  lds_order();
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int i = lane + 64 * t;
+    if (i < D) {
+      ss[i] = sc[t];
+      xs[i] = xv[t];
+    }
+  }
+  lds_order();
+  bool accepted = false;
+  if (!any_bad) {
+    // A^ in FP16: tile (I, J) writes rows 16I+4g+r at column 16J+c and, for
+    // I < J, row 16J+c at columns 16I+4g..+3 (one 8-byte store).
+#pragma unroll
+    for (int I = 0; I < NB; ++I) {
+      const f4 sr = *reinterpret_cast<const f4*>(ss + 16 * I + 4 * g);
+#pragma unroll
+      for (int J = I; J < NB; ++J) {
+        const int j = 16 * J + c;
+        const float scj = ss[j];
+        const f4 v = acc[tile_index(I, J, NB)];
+        h4 tv;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tv[r] = (_Float16)(v[r] * sr[r] * scj);
+        if (j < f8) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (16 * I + 4 * g + r < f) Ah[(16 * I + 4 * g + r) * sa + j] = tv[r];
+        }
+        if (I < J && j < f && 16 * I + 4 * g < f8) *reinterpret_cast<h4*>(Ah + j * sa + 16 * I + 4 * g) = tv;
+      }
+    }
+    double bb = 0.0;
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) bb += (double)bv[t] * (double)bv[t];
+    bb = wave_sum_d(bb);
+    const double tol2 = (double)cp.tol * (double)cp.tol * bb;
+    const int nch = f8 / 8;
+    for (int round = 0;; ++round) {
+      // FP64 residual of the current x (xs holds it).
+      gram_matvec_f64<NB>(acc, xs, ys, lane);
+      float rh[RPL];
+      double rr = 0.0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int i = lane + 64 * t;
+        const double r = i < f ? (double)bv[t] - ys[i] : 0.0;
+        rr += r * r;
+        rh[t] = (float)r * sc[t];
+      }
+      rr = wave_sum_d(rr);
+      if (rr <= tol2) {
+        accepted = true;
+        break;
+      }
+      if (round == cp.rounds) break;
+      float mu = 0.0f;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) mu = fmaxf(mu, fabsf(rh[t]));
+      mu = wave_max(mu);
+      if (!(mu > 0.0f && mu < __builtin_huge_valf())) break;
+      const float imu = 1.0f / mu;
+      float dv[RPL], rv[RPL], pv[RPL], q[RPL];
+      float rho = 0.0f;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        rv[t] = rh[t] * imu;
+        pv[t] = rv[t];
+        dv[t] = 0.0f;
+        rho = fmaf(rv[t], rv[t], rho);
+      }
+      rho = wave_sum(rho);
+      for (int it = 0; it < cp.iters; ++it) {
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int i = lane + 64 * t;
+          if (i < f8) p16[i] = (_Float16)pv[t];
+        }
+        lds_order();
+        cg_matvec<NB>(Ah, p16, f, sa, nch, lane, q);
+        float pq = 0.0f;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) pq = fmaf(pv[t], q[t], pq);
+        pq = wave_sum(pq);
+        if (!(pq > 0.0f) || !(rho > 0.0f)) break;
+        const float alpha = rho / pq;
+        float rn = 0.0f;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          dv[t] = fmaf(alpha, pv[t], dv[t]);
+          rv[t] = fmaf(-alpha, q[t], rv[t]);
+          rn = fmaf(rv[t], rv[t], rn);
+        }
+        rn = wave_sum(rn);
+        const float beta = rn / rho;
+        rho = rn;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) pv[t] = fmaf(beta, pv[t], rv[t]);
+        lds_order();
+      }
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int i = lane + 64 * t;
+        xv[t] = fmaf(mu * sc[t], dv[t], xv[t]);
+        if (i < D) xs[i] = i < f ? xv[t] : 0.0f;
+      }
+      lds_order();
+    }
+  }
+  if (!accepted) {
+    lds_order();
+    ok = chol_solve<NB>(acc, f, bcol, lds, xrow, ldf, lane, bcol2, xb, ldb);
+    return false;
+  }
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int i = lane + 64 * t;
+    const float xo = i < f ? xv[t] : 0.0f;
+    if (i < ldf) xrow[i] = xo;
+    if (xb && i < ldb) xb[i] = (__bf16)xo;
+  }
+  ok = true;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
 template <int NB, bool BF>
@@ -405,7 +786,26 @@ __device__ __forceinline__ void gram_any(f4 (&acc)[num_tiles(NB)], const SweepAr
     gram_range<NB>(acc, a.col, a.val, a.theta, a.ldf, start, count, lds, lane);
 }
 
-template <int NB, bool BF>
+// Solves the system in registers for one row: Cholesky, or (CG) the
+// mixed-precision refinement with its Cholesky fallback.  Counts rows whose
+// factorisation failed and rows that fell back.
+template <int NB, bool CG>
+__device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepArgs& a, bool bf, float* lds,
+                                          float* xrow, __bf16* xb, int lane) {
+  const int bcol = bf ? a.f : (int)a.ldf;
+  const int bcol2 = bf ? a.f + 1 : -1;
+  bool ok;
+  if constexpr (CG) {
+    const CgParams cp{a.cg_iters, a.cg_rounds, a.cg_tol};
+    const bool accepted = cg_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb, cp, ok);
+    if (!accepted && lane == 0) atomicAdd(a.cg_fallback, 1);
+  } else {
+    ok = chol_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb);
+  }
+  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
+}
+
+template <int NB, bool BF, bool CG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_task_kernel(SweepArgs a) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x;
@@ -435,13 +835,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
   }
   add_ridge<NB>(acc, a.f, a.lambda * (float)t.count, lane);
   lds_order();
-  const bool ok = BF ? chol_solve<NB>(acc, a.f, a.f, lds, xrow, a.ldf, lane, a.f + 1, xb, a.ldb)
-                     : chol_solve<NB>(acc, a.f, (int)a.ldf, lds, xrow, a.ldf, lane);
-  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
+  solve_row<NB, CG>(acc, a, BF, lds, xrow, xb, lane);
 }
 
-template <int NB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void als_finish_kernel(SweepArgs a) {
+template <int NB, bool CG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_finish_kernel(SweepArgs a) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x;
   const HeavyRow h = a.heavy[blockIdx.x];
@@ -460,9 +858,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void al
   float* xrow = a.x + (int64_t)h.row * a.ldf;
   const bool bf = a.precision != 0;
   __bf16* xb = bf ? reinterpret_cast<__bf16*>(a.x_b) + (int64_t)h.row * a.ldb : nullptr;
-  const bool ok = bf ? chol_solve<NB>(acc, a.f, a.f, lds, xrow, a.ldf, lane, a.f + 1, xb, a.ldb)
-                     : chol_solve<NB>(acc, a.f, (int)a.ldf, lds, xrow, a.ldf, lane);
-  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
+  lds_order();
+  solve_row<NB, CG>(acc, a, bf, lds, xrow, xb, lane);
 }
 
 // Gram dump for tests: the full augmented (16NB)^2 matrix of each listed
@@ -603,16 +1000,23 @@ int sweep_nb(int f) {
 template <int NB>
 static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which, const int32_t* rows,
                             float* out, int64_t nrows) {
-  const size_t shmem = sizeof(float) * (size_t)lds_floats_nb(NB, a.f);
   const bool bf = a.precision != 0;
+  const bool cg = a.solver != 0;
+  const size_t shmem = sizeof(float) * (size_t)cmax(lds_floats_nb(NB, a.f), cg ? cg_lds_floats(NB, a.f) : 0);
   if (which == 0) {
     if (a.ntasks > 0) {
-      if (bf) hipLaunchKernelGGL((als_task_kernel<NB, true>), dim3((unsigned)a.ntasks), dim3(64), shmem, s, a);
-      else hipLaunchKernelGGL((als_task_kernel<NB, false>), dim3((unsigned)a.ntasks), dim3(64), shmem, s, a);
+      const dim3 grid((unsigned)a.ntasks);
+      if (bf && cg) hipLaunchKernelGGL((als_task_kernel<NB, true, true>), grid, dim3(64), shmem, s, a);
+      else if (bf) hipLaunchKernelGGL((als_task_kernel<NB, true, false>), grid, dim3(64), shmem, s, a);
+      else if (cg) hipLaunchKernelGGL((als_task_kernel<NB, false, true>), grid, dim3(64), shmem, s, a);
+      else hipLaunchKernelGGL((als_task_kernel<NB, false, false>), grid, dim3(64), shmem, s, a);
     }
   } else if (which == 1) {
-    if (a.nheavy > 0)
-      hipLaunchKernelGGL(als_finish_kernel<NB>, dim3((unsigned)a.nheavy), dim3(64), shmem, s, a);
+    if (a.nheavy > 0) {
+      const dim3 grid((unsigned)a.nheavy);
+      if (cg) hipLaunchKernelGGL((als_finish_kernel<NB, true>), grid, dim3(64), shmem, s, a);
+      else hipLaunchKernelGGL((als_finish_kernel<NB, false>), grid, dim3(64), shmem, s, a);
+    }
   } else {
     if (nrows > 0) {
       if (bf) hipLaunchKernelGGL((als_dump_kernel<NB, true>), dim3((unsigned)nrows), dim3(64), shmem, s, a, rows, out);

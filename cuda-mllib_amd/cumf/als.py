@@ -19,12 +19,16 @@ from . import _paths
 CUMF_OPTIONS_VERSION = 1
 CUMF_PREC_FP32 = 0
 CUMF_PREC_BF16_GRAM = 1
+CUMF_SOLVER_CHOLESKY = 0
+CUMF_SOLVER_CG = 1
 
 
 class Options(C.Structure):
     _fields_ = [("version", C.c_int32), ("precision", C.c_int32), ("split", C.c_int32),
                 ("timing", C.c_int32), ("use_graph", C.c_int32),
-                ("compat_hermitian_bug", C.c_int32), ("reserved", C.c_int32 * 10)]
+                ("compat_hermitian_bug", C.c_int32), ("solver", C.c_int32),
+                ("cg_iters", C.c_int32), ("cg_rounds", C.c_int32), ("cg_tol", C.c_float),
+                ("reserved", C.c_int32 * 6)]
 
 
 class Stats(C.Structure):
@@ -33,7 +37,8 @@ class Stats(C.Structure):
                 ("rmse_ms", C.c_double), ("rmse_launches", C.c_int64),
                 ("sweep_ms", C.c_double), ("sweeps", C.c_int64),
                 ("failed_rows", C.c_int64), ("gram_flops", C.c_int64),
-                ("iter_ms", C.c_double), ("iters", C.c_int64)]
+                ("iter_ms", C.c_double), ("iters", C.c_int64),
+                ("cg_fallback_rows", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

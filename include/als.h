@@ -74,6 +74,14 @@ void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, f
                                    copies of the factors, b from a BF16 hi/lo split
                                    of the ratings, FP32 Cholesky */
 
+#define CUMF_SOLVER_CHOLESKY 0  /* FP32 Cholesky of every row's system */
+#define CUMF_SOLVER_CG 1        /* FP16 conjugate gradients on the Jacobi-scaled
+                                   system inside FP32 iterative refinement, the
+                                   residual b - A x formed in FP64, started from
+                                   the row's current factor; rows whose relative
+                                   residual stays above cg_tol fall back to the
+                                   Cholesky solve */
+
 typedef struct cumf_options {
   int32_t version;      /* CUMF_OPTIONS_VERSION */
   int32_t precision;    /* CUMF_PREC_* */
@@ -90,7 +98,11 @@ typedef struct cumf_options {
                            f % 10 == 0; slow path, for parity with the
                            reference's factors.  The legacy doALSWithCSR
                            takes it from CUMF_COMPAT_HERMITIAN_BUG=1. */
-  int32_t reserved[10];
+  int32_t solver;       /* CUMF_SOLVER_* */
+  int32_t cg_iters;     /* CG steps per refinement round (0: 6) */
+  int32_t cg_rounds;    /* refinement rounds (0: 2) */
+  float cg_tol;         /* accept ||b - A x|| <= cg_tol ||b|| (0: 1e-3) */
+  int32_t reserved[6];
 } cumf_options;
 
 typedef struct cumf_stats {
@@ -106,6 +118,7 @@ typedef struct cumf_stats {
   int64_t gram_flops;      /* 2 nnz f(f+1) summed over the half-sweeps run */
   double iter_ms;          /* graph replays of whole iterations (use_graph) */
   int64_t iters;
+  int64_t cg_fallback_rows; /* CUMF_SOLVER_CG: rows solved again by Cholesky */
 } cumf_stats;
 
 typedef struct cumf_handle cumf_handle;
