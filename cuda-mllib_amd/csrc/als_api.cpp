@@ -21,9 +21,14 @@
 #include <vector>
 
 #include "../../include/als.h"
+#include "../../include/cumf_mlp.h"
 #include "../../include/utilities.h"
 #include "als_kernels.h"
 #include "cumf_internal.h"
+#include "mlp_internal.h"
+#include "mlp_kernels.h"
+
+using cumf::DevBuf;
 
 namespace {
 
@@ -34,40 +39,6 @@ constexpr int kDefaultCgRounds = 2;
 constexpr float kDefaultCgTol = 1e-3f;
 
 int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }
-
-// Device allocation owned by the host driver.
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { reset(); }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  int alloc(size_t n) {
-    reset();
-    if (n == 0) return CUMF_OK;
-    if (hipMalloc(&p, n) != hipSuccess) {
-      (void)hipGetLastError();
-      p = nullptr;
-      cumf::set_last_error("hipMalloc of " + std::to_string(n) + " bytes failed");
-      return CUMF_ERR_NOMEM;
-    }
-    bytes = n;
-    return CUMF_OK;
-  }
-  template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-#define CUMF_TRY(expr)          \
-  do {                          \
-    int rc_ = (expr);           \
-    if (rc_ != CUMF_OK) return rc_; \
-  } while (0)
 
 // Wavefront task list of one rating matrix side.
 struct Plan {
@@ -122,7 +93,7 @@ int check_f(int f) {
 
 // Kernel-time bookkeeping with HIP events.
 struct Timer {
-  enum Kind { TASK = 0, FINISH, RMSE, SWEEP, ITER, NKINDS };
+  enum Kind { TASK = 0, FINISH, RMSE, SWEEP, ITER, WARM, NKINDS };
   struct Pending { hipEvent_t a, b; int kind; };
   std::vector<hipEvent_t> pool;
   std::vector<Pending> pending;
@@ -173,6 +144,10 @@ struct cumf_handle {
   Timer timer;
   int64_t gram_flops = 0;
   bool have_ratings = false;
+  // Learned warm start: attached network, features and predictions of the
+  // X side's rows.
+  cumf_mlp* mlp = nullptr;
+  DevBuf feat, x0;
   // One captured iteration (use_graph), rebuilt after cumf_set_ratings.
   hipGraphExec_t iter_graph = nullptr;
   bool capturing = false;
@@ -220,8 +195,28 @@ int timed(cumf_handle* h, int kind, F&& launch) {
   return CUMF_OK;
 }
 
-int run_half_sweep(cumf_handle* h, const Plan& p, const DevBuf& rowp, const DevBuf& col,
-                   const DevBuf& val, const float* theta, float* x, const DevBuf& theta_b, DevBuf& x_b) {
+bool uses_network(const cumf_handle* h) {
+  return h->opt.solver == CUMF_SOLVER_CG &&
+         (h->opt.warm_start == CUMF_WARM_MLP || h->opt.warm_start == CUMF_WARM_BEST);
+}
+
+// Buffers of the learned warm start (allocated outside any capture).
+int prepare_network(cumf_handle* h) {
+  if (!uses_network(h)) return CUMF_OK;
+  if (!h->mlp) {
+    cumf::set_last_error("warm_start CUMF_WARM_MLP / CUMF_WARM_BEST needs a network (cumf_set_mlp)");
+    return CUMF_ERR_STATE;
+  }
+  const size_t fb = 2 * (size_t)h->m * cumf::mlp_k0(h->mlp);
+  if (h->feat.bytes < fb) CUMF_TRY(h->feat.alloc(fb));
+  const size_t xb = sizeof(float) * (size_t)h->m * h->ldf;
+  if (h->x0.bytes < xb) CUMF_TRY(h->x0.alloc(xb));
+  return cumf::mlp_prepare_predict(h->mlp);
+}
+
+int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, const DevBuf& col,
+                   const DevBuf& val, const float* theta, float* x, const DevBuf& theta_b, DevBuf& x_b,
+                   float* trace = nullptr) {
   if (h->opt.compat_hermitian_bug) {
     cumf::CompatArgs c{};
     c.row_ptr = rowp.as<int32_t>();
@@ -262,6 +257,24 @@ int run_half_sweep(cumf_handle* h, const Plan& p, const DevBuf& rowp, const DevB
   a.cg_rounds = h->opt.cg_rounds;
   a.cg_tol = h->opt.cg_tol;
   a.cg_fallback = h->fail.as<int>() + 1;
+  a.cg_trace = trace;
+  a.warm = h->opt.warm_start;
+  if (side == 1 && a.warm >= CUMF_WARM_MLP) a.warm = CUMF_WARM_PREV;  // the network predicts X rows
+  if (side == 0 && uses_network(h)) {
+    CUMF_TRY(prepare_network(h));
+    const int K0 = cumf::mlp_k0(h->mlp);
+    CUMF_TRY(timed(h, Timer::WARM, [&] {
+      hipError_t e = cumf::launch_mlp_features(rowp.as<int32_t>(), col.as<int32_t>(), val.as<float>(), theta,
+                                               h->ldf, h->f, h->lambda, 0, p.rows, K0, h->feat.as<__bf16>(),
+                                               h->stream);
+      if (e != hipSuccess) return e;
+      return cumf::mlp_predict_device(h->mlp, h->feat.as<__bf16>(), p.rows, h->x0.as<float>(), h->ldf,
+                                      h->stream) == CUMF_OK
+                 ? hipSuccess
+                 : hipErrorLaunchFailure;
+    }));
+    a.x0 = h->x0.as<float>();
+  }
   hipEvent_t s0 = nullptr, s1 = nullptr;
   if (h->opt.timing && !h->capturing) {
     s0 = h->timer.get();
@@ -334,6 +347,11 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
   }
   if (o.solver != CUMF_SOLVER_CHOLESKY && o.compat_hermitian_bug) {
     cumf::set_last_error("compat_hermitian_bug reproduces the reference's LU solve; use CUMF_SOLVER_CHOLESKY");
+    return CUMF_ERR_INVALID;
+  }
+  if (o.warm_start < CUMF_WARM_PREV || o.warm_start > CUMF_WARM_BEST ||
+      (o.warm_start != CUMF_WARM_PREV && o.solver != CUMF_SOLVER_CG)) {
+    cumf::set_last_error("warm_start must be a CUMF_WARM_* value; only CUMF_SOLVER_CG has a start");
     return CUMF_ERR_INVALID;
   }
   if (o.cg_iters > 64 || o.cg_rounds > 16) {
@@ -552,7 +570,7 @@ int cumf_update_x(cumf_handle* h) {
     return CUMF_ERR_STATE;
   }
   DeviceGuard dg(h->device);
-  return run_half_sweep(h, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb, h->Xb);
+  return run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb, h->Xb);
 }
 
 int cumf_update_theta(cumf_handle* h) {
@@ -561,21 +579,24 @@ int cumf_update_theta(cumf_handle* h) {
     return CUMF_ERR_STATE;
   }
   DeviceGuard dg(h->device);
-  return run_half_sweep(h, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb, h->Tb);
+  return run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb, h->Tb);
 }
 
 namespace {
 
-// Captures one iteration (both half-sweeps: four kernel launches, no
-// memsets or host work) on the handle's stream into an executable graph.
+// Captures one iteration (both half-sweeps' kernel launches, including the
+// warm start's features and network, no memsets or host work) on the
+// handle's stream into an executable graph.
 int capture_iteration(cumf_handle* h) {
   h->drop_graph();
+  CUMF_TRY(prepare_network(h));
   hipGraph_t g = nullptr;
   CUMF_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   h->capturing = true;
   const int64_t flops = h->gram_flops;
-  int rc = run_half_sweep(h, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb, h->Xb);
-  if (rc == CUMF_OK) rc = run_half_sweep(h, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb, h->Tb);
+  int rc = run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb, h->Xb);
+  if (rc == CUMF_OK)
+    rc = run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb, h->Tb);
   h->capturing = false;
   h->gram_flops = flops;  // counted again at each replay
   hipError_t e = hipStreamEndCapture(h->stream, &g);
@@ -670,6 +691,8 @@ int cumf_get_stats(cumf_handle* h, cumf_stats* s) {
   s->sweeps = h->timer.count[Timer::SWEEP];
   s->failed_rows = fails[0];
   s->cg_fallback_rows = fails[1];
+  s->warm_ms = h->timer.ms[Timer::WARM];
+  s->warm_launches = h->timer.count[Timer::WARM];
   s->gram_flops = h->gram_flops;
   s->iter_ms = h->timer.ms[Timer::ITER];
   s->iters = h->timer.count[Timer::ITER];
@@ -731,6 +754,123 @@ int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t n
   }
   CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
   CUMF_HIP_TRY(hipMemcpy(out, dout.p, dout.bytes, hipMemcpyDeviceToHost));
+  return CUMF_OK;
+}
+
+int cumf_debug_cg_trace(cumf_handle* h, int side, float* out) {
+  cumf::clear_last_error();
+  if (!h || !out || (side != 0 && side != 1)) return CUMF_ERR_INVALID;
+  if (!h->have_ratings) {
+    cumf::set_last_error("cumf_debug_cg_trace: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  DeviceGuard dg(h->device);
+  const int64_t rows = side == 0 ? h->m : h->n;
+  DevBuf tr;
+  CUMF_TRY(tr.alloc(sizeof(float) * 4 * (size_t)rows));
+  CUMF_HIP_TRY(hipMemsetAsync(tr.p, 0, tr.bytes, h->stream));
+  if (side == 0)
+    CUMF_TRY(run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb,
+                            h->Xb, tr.as<float>()));
+  else
+    CUMF_TRY(run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb,
+                            h->Tb, tr.as<float>()));
+  CUMF_HIP_TRY(hipMemcpyAsync(out, tr.p, tr.bytes, hipMemcpyDeviceToHost, h->stream));
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  return CUMF_OK;
+}
+
+int cumf_set_mlp(cumf_handle* h, cumf_mlp* mlp) {
+  cumf::clear_last_error();
+  if (!h) return CUMF_ERR_INVALID;
+  if (mlp && (cumf::mlp_f(mlp) != h->f || cumf::mlp_device(mlp) != h->device)) {
+    cumf::set_last_error("cumf_set_mlp: network built for another rank or device");
+    return CUMF_ERR_INVALID;
+  }
+  DeviceGuard dg(h->device);
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  h->mlp = mlp;
+  h->drop_graph();
+  return CUMF_OK;
+}
+
+int cumf_mlp_harvest(cumf_handle* h, cumf_mlp* mlp, int iters) {
+  cumf::clear_last_error();
+  if (!h || !mlp || iters < 0) return CUMF_ERR_INVALID;
+  if (!h->have_ratings) {
+    cumf::set_last_error("cumf_mlp_harvest: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  if (cumf::mlp_f(mlp) != h->f || cumf::mlp_device(mlp) != h->device) {
+    cumf::set_last_error("cumf_mlp_harvest: network built for another rank or device");
+    return CUMF_ERR_INVALID;
+  }
+  DeviceGuard dg(h->device);
+  // The plain solver produces the targets.
+  const cumf_options saved = h->opt;
+  h->opt.solver = CUMF_SOLVER_CHOLESKY;
+  h->opt.warm_start = CUMF_WARM_PREV;
+  int rc = CUMF_OK;
+  for (int it = 0; it < iters && rc == CUMF_OK; ++it) {
+    __bf16* feats = nullptr;
+    float* targets = nullptr;
+    if (hipStreamSynchronize(h->stream) != hipSuccess) {
+      rc = CUMF_ERR_HIP;
+      break;
+    }
+    rc = cumf::mlp_harvest_reserve(mlp, h->m, &feats, &targets);
+    if (rc != CUMF_OK) break;
+    if (cumf::launch_mlp_features(h->rp.as<int32_t>(), h->ci.as<int32_t>(), h->rv.as<float>(), h->T.as<float>(),
+                                  h->ldf, h->f, h->lambda, 0, h->m, cumf::mlp_k0(mlp), feats,
+                                  h->stream) != hipSuccess) {
+      rc = CUMF_ERR_HIP;
+      break;
+    }
+    rc = run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb, h->Xb);
+    if (rc != CUMF_OK) break;
+    if (hipMemcpy2DAsync(targets, sizeof(float) * h->f, h->X.p, sizeof(float) * h->ldf, sizeof(float) * h->f,
+                         h->m, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) {
+      rc = CUMF_ERR_HIP;
+      break;
+    }
+    rc = run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb, h->Tb);
+  }
+  h->opt = saved;
+  if (rc == CUMF_ERR_HIP) cumf::set_last_error("cumf_mlp_harvest: HIP error");
+  if (rc == CUMF_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = CUMF_ERR_HIP;
+  return rc;
+}
+
+int cumf_debug_features(cumf_handle* h, int side, const int32_t* rows, int64_t nrows, float* out) {
+  cumf::clear_last_error();
+  if (!h || !rows || !out || nrows < 0 || (side != 0 && side != 1)) return CUMF_ERR_INVALID;
+  if (!h->have_ratings) {
+    cumf::set_last_error("cumf_debug_features: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  DeviceGuard dg(h->device);
+  const int K0 = (2 * h->f + 1 + 31) / 32 * 32;
+  const int64_t total = side == 0 ? h->m : h->n;
+  for (int64_t k = 0; k < nrows; ++k)
+    if (rows[k] < 0 || rows[k] >= total) return CUMF_ERR_INVALID;
+  DevBuf fb;
+  CUMF_TRY(fb.alloc(2 * (size_t)total * K0));
+  const DevBuf& rp = side == 0 ? h->rp : h->cp;
+  const DevBuf& ci = side == 0 ? h->ci : h->ri;
+  const DevBuf& cv = side == 0 ? h->rv : h->cv;
+  const float* th = side == 0 ? h->T.as<float>() : h->X.as<float>();
+  CUMF_HIP_TRY(cumf::launch_mlp_features(rp.as<int32_t>(), ci.as<int32_t>(), cv.as<float>(), th, h->ldf, h->f,
+                                         h->lambda, 0, total, K0, fb.as<__bf16>(), h->stream));
+  std::vector<uint16_t> host((size_t)total * K0);
+  CUMF_HIP_TRY(hipMemcpyAsync(host.data(), fb.p, fb.bytes, hipMemcpyDeviceToHost, h->stream));
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int64_t k = 0; k < nrows; ++k)
+    for (int i = 0; i < K0; ++i) {
+      const uint32_t bits = (uint32_t)host[(size_t)rows[k] * K0 + i] << 16;
+      float v;
+      std::memcpy(&v, &bits, 4);
+      out[k * K0 + i] = v;
+    }
   return CUMF_OK;
 }
 

@@ -55,6 +55,11 @@ struct SweepArgs {
   int cg_rounds;
   float cg_tol;
   int* cg_fallback;        // rows that fell back to Cholesky
+  // CG start: 0 the current factor, 1 zero, 2 x0 (row stride ldf), 3 the
+  // better of the current factor and x0 by FP64 residual.
+  int warm;
+  const float* x0;
+  float* cg_trace;         // optional, 4 floats per row: initial / final relative residual, rounds, fell back
 };
 
 // The reference-reproducing path (compat.hip): one work-group per row.

@@ -595,14 +595,33 @@ struct CgParams {
   int iters;   // CG iterations per refinement round
   int rounds;  // refinement rounds
   float tol;   // accept x once ||b - A x|| <= tol ||b||
+  int warm;    // start: 0 the row's current factor, 1 zero, 2 x0row, 3 the better of 0 and 2
 };
+
+// Squared FP64 residual norm of the x in xs (lane-per-row b in bv).
+template <int NB>
+__device__ __forceinline__ double residual_norm2(const f4 (&acc)[num_tiles(NB)], const float* xs, double* ys,
+                                                 const float (&bv)[rows_per_lane(NB)], int f, int lane) {
+  gram_matvec_f64<NB>(acc, xs, ys, lane);
+  double rr = 0.0;
+#pragma unroll
+  for (int t = 0; t < rows_per_lane(NB); ++t) {
+    const int i = lane + 64 * t;
+    const double r = i < f ? (double)bv[t] - ys[i] : 0.0;
+    rr += r * r;
+  }
+  return wave_sum_d(rr);
+}
 
 // Returns 1 if x was accepted, 0 if the Cholesky fallback ran (ok reports
 // the fallback's pivots), and writes x to xrow / xb as chol_solve does.
+// x0row: the network's prediction for this row (warm 2 and 3).  trace
+// (optional): initial and final relative residual, rounds run, fell back.
 template <int NB>
 __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bcol, float* __restrict__ lds,
                                          float* __restrict__ xrow, int64_t ldf, int lane, int bcol2,
-                                         __bf16* __restrict__ xb, int64_t ldb, const CgParams& cp, bool& ok) {
+                                         __bf16* __restrict__ xb, int64_t ldb, const CgParams& cp,
+                                         const float* __restrict__ x0row, float* __restrict__ trace, bool& ok) {
   constexpr int RPL = rows_per_lane(NB);
   constexpr int D = 16 * NB;
   const int g = lane >> 4, c = lane & 15;
@@ -631,8 +650,8 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
         if (j == bcol2 && i < f) bs[D + i] = v;
       }
   lds_order();
-  float sc[RPL], bv[RPL], xv[RPL];
-  bool bad = false;
+  float sc[RPL], bv[RPL], xv[RPL], xm[RPL];
+  bool bad = false, mbad = false;
 #pragma unroll
   for (int t = 0; t < RPL; ++t) {
     const int i = lane + 64 * t;
@@ -642,11 +661,20 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
     bad = bad || !good;
     sc[t] = (in && good) ? __builtin_amdgcn_rsqf(d) : 0.0f;
     bv[t] = in ? bs[i] + (bcol2 >= 0 ? bs[D + i] : 0.0f) : 0.0f;
-    float x0 = in ? xrow[i] : 0.0f;
+    float x0 = (in && cp.warm != 1) ? xrow[i] : 0.0f;
     if (!(fabsf(x0) < __builtin_huge_valf())) x0 = 0.0f;
     xv[t] = x0;
+    xm[t] = (in && cp.warm >= 2) ? x0row[i] : 0.0f;
+    mbad = mbad || !(fabsf(xm[t]) < __builtin_huge_valf());
   }
-  const bool any_bad = __any(bad);
+  const bool net_bad = cp.warm >= 2 && __any(mbad);
+  // A non-finite prediction is never used: with warm 2 the row goes to the
+  // Cholesky solve, with warm 3 the current factor is the start.
+  const bool any_bad = __any(bad) || (cp.warm == 2 && net_bad);
+  if (cp.warm == 2 && !net_bad) {
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) xv[t] = xm[t];
+  }
   lds_order();
 #pragma unroll
   for (int t = 0; t < RPL; ++t) {
@@ -658,6 +686,8 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
   }
   lds_order();
   bool accepted = false;
+  float rel0 = __builtin_nanf(""), rel = __builtin_nanf("");
+  int rounds_run = 0;
   if (!any_bad) {
     // A^ in FP16: tile (I, J) writes rows 16I+4g+r at column 16J+c and, for
     // I < J, row 16J+c at columns 16I+4g..+3 (one 8-byte store).
@@ -686,6 +716,25 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
     bb = wave_sum_d(bb);
     const double tol2 = (double)cp.tol * (double)cp.tol * bb;
     const int nch = f8 / 8;
+    if (cp.warm == 3 && !net_bad) {
+      // Start from the better of the current factor (in xs) and the prediction.
+      const double rp = residual_norm2<NB>(acc, xs, ys, bv, f, lane);
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int i = lane + 64 * t;
+        if (i < D) xs[i] = i < f ? xm[t] : 0.0f;
+      }
+      lds_order();
+      const double rm = residual_norm2<NB>(acc, xs, ys, bv, f, lane);
+      const bool use_net = rm < rp;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int i = lane + 64 * t;
+        if (use_net) xv[t] = xm[t];
+        if (i < D) xs[i] = i < f ? xv[t] : 0.0f;
+      }
+      lds_order();
+    }
     for (int round = 0;; ++round) {
       // FP64 residual of the current x (xs holds it).
       gram_matvec_f64<NB>(acc, xs, ys, lane);
@@ -699,6 +748,9 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
         rh[t] = (float)r * sc[t];
       }
       rr = wave_sum_d(rr);
+      if (round == 0) rel0 = (float)sqrt(rr / bb);
+      rel = (float)sqrt(rr / bb);
+      rounds_run = round;
       if (rr <= tol2) {
         accepted = true;
         break;
@@ -757,6 +809,12 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
       lds_order();
     }
   }
+  if (trace && lane == 0) {
+    trace[0] = rel0;
+    trace[1] = rel;
+    trace[2] = (float)rounds_run;
+    trace[3] = accepted ? 0.0f : 1.0f;
+  }
   if (!accepted) {
     lds_order();
     ok = chol_solve<NB>(acc, f, bcol, lds, xrow, ldf, lane, bcol2, xb, ldb);
@@ -791,13 +849,17 @@ __device__ __forceinline__ void gram_any(f4 (&acc)[num_tiles(NB)], const SweepAr
 // factorisation failed and rows that fell back.
 template <int NB, bool CG>
 __device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepArgs& a, bool bf, float* lds,
-                                          float* xrow, __bf16* xb, int lane) {
+                                          int64_t row, __bf16* xb, int lane) {
+  float* xrow = a.x + row * a.ldf;
   const int bcol = bf ? a.f : (int)a.ldf;
   const int bcol2 = bf ? a.f + 1 : -1;
   bool ok;
   if constexpr (CG) {
-    const CgParams cp{a.cg_iters, a.cg_rounds, a.cg_tol};
-    const bool accepted = cg_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb, cp, ok);
+    const CgParams cp{a.cg_iters, a.cg_rounds, a.cg_tol, a.warm};
+    const float* x0row = a.x0 ? a.x0 + row * a.ldf : nullptr;
+    float* trace = a.cg_trace ? a.cg_trace + 4 * row : nullptr;
+    const bool accepted =
+        cg_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb, cp, x0row, trace, ok);
     if (!accepted && lane == 0) atomicAdd(a.cg_fallback, 1);
   } else {
     ok = chol_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb);
@@ -835,7 +897,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
   }
   add_ridge<NB>(acc, a.f, a.lambda * (float)t.count, lane);
   lds_order();
-  solve_row<NB, CG>(acc, a, BF, lds, xrow, xb, lane);
+  solve_row<NB, CG>(acc, a, BF, lds, t.row, xb, lane);
 }
 
 template <int NB, bool CG>
@@ -855,11 +917,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
   }
   const int n_u = a.row_ptr[h.row + 1] - a.row_ptr[h.row];
   add_ridge<NB>(acc, a.f, a.lambda * (float)n_u, lane);
-  float* xrow = a.x + (int64_t)h.row * a.ldf;
   const bool bf = a.precision != 0;
   __bf16* xb = bf ? reinterpret_cast<__bf16*>(a.x_b) + (int64_t)h.row * a.ldb : nullptr;
   lds_order();
-  solve_row<NB, CG>(acc, a, bf, lds, xrow, xb, lane);
+  solve_row<NB, CG>(acc, a, bf, lds, h.row, xb, lane);
 }
 
 // Gram dump for tests: the full augmented (16NB)^2 matrix of each listed

@@ -21,6 +21,10 @@ CUMF_PREC_FP32 = 0
 CUMF_PREC_BF16_GRAM = 1
 CUMF_SOLVER_CHOLESKY = 0
 CUMF_SOLVER_CG = 1
+CUMF_WARM_PREV = 0
+CUMF_WARM_ZERO = 1
+CUMF_WARM_MLP = 2
+CUMF_WARM_BEST = 3
 
 
 class Options(C.Structure):
@@ -28,7 +32,7 @@ class Options(C.Structure):
                 ("timing", C.c_int32), ("use_graph", C.c_int32),
                 ("compat_hermitian_bug", C.c_int32), ("solver", C.c_int32),
                 ("cg_iters", C.c_int32), ("cg_rounds", C.c_int32), ("cg_tol", C.c_float),
-                ("reserved", C.c_int32 * 6)]
+                ("warm_start", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class Stats(C.Structure):
@@ -38,7 +42,7 @@ class Stats(C.Structure):
                 ("sweep_ms", C.c_double), ("sweeps", C.c_int64),
                 ("failed_rows", C.c_int64), ("gram_flops", C.c_int64),
                 ("iter_ms", C.c_double), ("iters", C.c_int64),
-                ("cg_fallback_rows", C.c_int64)]
+                ("cg_fallback_rows", C.c_int64), ("warm_ms", C.c_double), ("warm_launches", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -87,6 +91,21 @@ def lib():
         "cumf_debug_compat_system": ([P, C.c_int, P, C.c_int64, P], C.c_int),
         "cumf_debug_copy": ([C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "cumf_debug_force_host_alloc": ([C.c_int], C.c_int),
+        "cumf_debug_cg_trace": ([P, C.c_int, P], C.c_int),
+        "cumf_debug_features": ([P, C.c_int, P, C.c_int64, P], C.c_int),
+        "cumf_set_mlp": ([P, P], C.c_int),
+        "cumf_mlp_create": ([C.c_int, C.c_int, C.c_uint64, C.POINTER(P)], C.c_int),
+        "cumf_mlp_destroy": ([P], C.c_int),
+        "cumf_mlp_dims": ([P, P, C.POINTER(C.c_int64)], C.c_int),
+        "cumf_mlp_get_params": ([P, P], C.c_int),
+        "cumf_mlp_set_params": ([P, P], C.c_int),
+        "cumf_mlp_predict_host": ([P, P, C.c_int64, P], C.c_int),
+        "cumf_mlp_gradients_host": ([P, P, P, C.c_int64, P, C.POINTER(C.c_double)], C.c_int),
+        "cumf_mlp_step_host": ([P, P, P, C.c_int64, C.c_float, C.POINTER(C.c_double)], C.c_int),
+        "cumf_mlp_harvest": ([P, P, C.c_int], C.c_int),
+        "cumf_mlp_harvested": ([P], C.c_int64),
+        "cumf_mlp_clear_harvest": ([P], C.c_int),
+        "cumf_mlp_train": ([P, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_double)], C.c_int),
         "cumf_memcpy_h2d": ([P, P, C.c_size_t], C.c_int),
         "cumf_memcpy_d2h": ([P, P, C.c_size_t], C.c_int),
         "cumf_malloc_best": ([C.POINTER(P), C.c_size_t], C.c_int),
@@ -227,6 +246,27 @@ class ALS:
                "cumf_debug_compat_system")
         M = out[:, :self.f * self.f].reshape(-1, self.f, self.f)
         return M, out[:, self.f * self.f:]
+
+    def cg_trace(self, side):
+        """One half-sweep (side 0 = X, 1 = Theta) recording per row: initial and
+        final relative residual, refinement rounds, fell back (0/1)."""
+        rows = self.m if side == 0 else self.n
+        out = np.zeros((rows, 4), np.float32)
+        _check(lib().cumf_debug_cg_trace(self._h, side, _ptr(out)), "cumf_debug_cg_trace")
+        return out
+
+    def features(self, side, rows):
+        """FP32 values of the BF16 warm-start features of listed rows."""
+        rows = np.ascontiguousarray(rows, np.int32)
+        k0 = (2 * self.f + 1 + 31) // 32 * 32
+        out = np.empty((rows.shape[0], k0), np.float32)
+        _check(lib().cumf_debug_features(self._h, side, _ptr(rows), rows.shape[0], _ptr(out)),
+               "cumf_debug_features")
+        return out
+
+    def set_mlp(self, mlp):
+        _check(lib().cumf_set_mlp(self._h, None if mlp is None else mlp._h), "cumf_set_mlp")
+        self._mlp = mlp  # keeps the network alive while attached
 
     def row_gram(self, side, rows):
         """Augmented Gram (D x D, ridge included) of listed rows; side 0 = X."""

@@ -82,6 +82,13 @@ void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, f
                                    residual stays above cg_tol fall back to the
                                    Cholesky solve */
 
+#define CUMF_WARM_PREV 0        /* CG starts from the row's current factor */
+#define CUMF_WARM_ZERO 1        /* CG starts from zero */
+#define CUMF_WARM_MLP 2         /* X half-sweeps start from the attached network's
+                                   prediction (cumf_mlp.h); Theta half-sweeps
+                                   from the current factor */
+#define CUMF_WARM_BEST 3        /* the better of PREV and MLP by FP64 residual */
+
 typedef struct cumf_options {
   int32_t version;      /* CUMF_OPTIONS_VERSION */
   int32_t precision;    /* CUMF_PREC_* */
@@ -102,7 +109,8 @@ typedef struct cumf_options {
   int32_t cg_iters;     /* CG steps per refinement round (0: 6) */
   int32_t cg_rounds;    /* refinement rounds (0: 2) */
   float cg_tol;         /* accept ||b - A x|| <= cg_tol ||b|| (0: 1e-3) */
-  int32_t reserved[6];
+  int32_t warm_start;   /* CUMF_WARM_* (CG only) */
+  int32_t reserved[5];
 } cumf_options;
 
 typedef struct cumf_stats {
@@ -119,6 +127,8 @@ typedef struct cumf_stats {
   double iter_ms;          /* graph replays of whole iterations (use_graph) */
   int64_t iters;
   int64_t cg_fallback_rows; /* CUMF_SOLVER_CG: rows solved again by Cholesky */
+  double warm_ms;          /* features + network predictions (CUMF_WARM_MLP/BEST) */
+  int64_t warm_launches;
 } cumf_stats;
 
 typedef struct cumf_handle cumf_handle;
@@ -205,6 +215,12 @@ int cumf_debug_copy(int device, int64_t bytes, int iters, double* ms_per_launch)
  * side 0 = X (CSR), 1 = Theta (CSC).  out: host. */
 int cumf_debug_compat_system(cumf_handle* h, int side, const int32_t* rows, int64_t nrows,
                              double* out);
+/* Runs one half-sweep of side 0 (X) or 1 (Theta) as cumf_update_x /
+ * cumf_update_theta do, recording for each row solved by CUMF_SOLVER_CG:
+ * out[4 u .. 4 u + 3] = initial and final relative residual
+ * ||b - A x|| / ||b|| (FP64, NaN if CG did not start), refinement rounds
+ * run, 1 if the row fell back to Cholesky.  out: host, rows x 4. */
+int cumf_debug_cg_trace(cumf_handle* h, int side, float* out);
 /* 1: mallocBest skips device memory and takes its mapped-host fallback
  * (also set by the environment variable CUMF_FORCE_HOST_ALLOC=1). */
 int cumf_debug_force_host_alloc(int on);

@@ -9,7 +9,7 @@ to a predicted solution x_u (f values):
     H0 = bf16(features)                    zero-padded to K0 = round_up(2f+1, 32)
     Z1 = H0 W1^T + c1,  H1 = bf16(gelu(Z1))           (hidden = 256)
     Z2 = H1 W2^T + c2,  H2 = bf16(gelu(Z2))
-    Y  = H2 W3^T + c3                                 (N3 = round_up(f, 64) rows of W3)
+    Y  = H2 W3^T + c3                                 (N3 = round_up(f, 32) rows of W3)
 with BF16 copies of the weights as GEMM operands and FP32 accumulation;
 gelu(z) = z Phi(z) (erf form).  Loss: mean over the batch and the f real
 outputs of (Y - x*)^2.  Backward (the rounding points the kernels use):
@@ -32,7 +32,7 @@ HIDDEN = 256
 
 def dims(f):
     k0 = (2 * f + 1 + 31) // 32 * 32
-    n3 = (f + 63) // 64 * 64
+    n3 = (f + 31) // 32 * 32
     return k0, HIDDEN, HIDDEN, n3
 
 

@@ -1,0 +1,196 @@
+"""GPU checks of the learned warm start (include/cumf_mlp.h) against the
+FP64 numpy checker (oracle/mlp_oracle.py), through the C ABI.
+
+Bars:
+  * initial weights equal the checker's generator bit for bit;
+  * forward: ||Y_gpu - Y_64|| <= 5e-3 ||Y_64|| (the checker rounds to BF16
+    where the kernels do; the residue is FP32 vs FP64 accumulation moving an
+    occasional BF16 rounding);
+  * gradients: per tensor ||g_gpu - g_64|| <= 2e-2 ||g_64||, loss to 1e-4;
+  * Adam: one step on the GPU's own gradient matches the checker's update
+    to 1e-6 absolute;
+  * features: within 2^-7 relative of the FP64 features (BF16 storage);
+  * training on harvested sweeps lowers the loss; the network's start
+    beats a cold start on a run it was not trained on; a poisoned network
+    sends every row to the Cholesky solve, whose factors match the plain
+    solver's.
+"""
+import numpy as np
+import pytest
+
+import cumf
+from cumf import als as G
+from cumf import mlp as NN
+from oracle import mlp_oracle as M
+from tests import _data
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_batch(net, B, seed):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((B, net.k0), np.float32)
+    X[:, 0] = rng.uniform(0, 7, B)
+    X[:, 1:1 + net.f] = rng.uniform(0.05, 1.0, (B, net.f))
+    X[:, 1 + net.f:1 + 2 * net.f] = rng.normal(0, 0.5, (B, net.f))
+    T = rng.normal(0, 0.3, (B, net.f)).astype(np.float32)
+    return X, T
+
+
+@pytest.mark.parametrize("f", [32, 100])
+def test_init_and_forward_match_checker(f):
+    with NN.MLP(f, seed=7) as net:
+        P = net.get_params()
+        ref = M.init_params(f, 7)
+        for a, b in zip(P, ref):
+            np.testing.assert_array_equal(a, b)
+        X, _ = _rand_batch(net, 300, 1)
+        Yg = net.predict(X)
+    Yr, _ = M.forward(ref, X.astype(np.float64))
+    err = np.linalg.norm(Yg - Yr[:, :f]) / np.linalg.norm(Yr[:, :f])
+    assert err <= 5e-3, err
+
+
+@pytest.mark.parametrize("f", [32, 100])
+def test_gradients_match_checker(f):
+    with NN.MLP(f, seed=3) as net:
+        # non-zero biases so their gradients matter
+        P = net.get_params()
+        rng = np.random.default_rng(5)
+        for i in (1, 3, 5):
+            P[i][:] = rng.normal(0, 0.05, P[i].shape).astype(np.float32)
+        P[5][f:] = 0.0
+        net.set_params(P)
+        X, T = _rand_batch(net, 512, 2)
+        grads, loss = net.gradients(X, T)
+    Tp = np.zeros((512, M.dims(f)[3]))
+    Tp[:, :f] = T
+    Y, cache = M.forward(P, X.astype(np.float64))
+    ref = M.backward(P, cache, Y, Tp, f)
+    assert abs(loss - M.loss(Y, Tp, f)) <= 1e-4 * M.loss(Y, Tp, f)
+    for k, (g, r) in enumerate(zip(grads, ref)):
+        err = np.linalg.norm(g - r) / np.linalg.norm(r)
+        assert err <= 2e-2, (k, err)
+
+
+def test_adam_step_matches_checker():
+    f = 32
+    with NN.MLP(f, seed=4) as net:
+        P0 = net.get_params()
+        X, T = _rand_batch(net, 256, 3)
+        grads, _ = net.gradients(X, T)
+        net.step(X, T, 1e-3)
+        P1 = net.get_params()
+    ref = [p.copy() for p in P0]
+    mm = [np.zeros_like(p) for p in P0]
+    vv = [np.zeros_like(p) for p in P0]
+    M.adam_step(ref, grads, mm, vv, 1, lr=1e-3)
+    for a, b in zip(P1, ref):
+        assert np.abs(a - b).max() <= 1e-6
+
+
+def test_features_match_checker():
+    ds = _data.netflix_like_small()
+    f, lam = 32, 0.058
+    T0 = cumf.init_factors(ds.n, f, 9)
+    h = G.ALS(f, lam)
+    h.set_dataset(ds)
+    h.set_factors(theta=T0)
+    deg = np.diff(ds.csr_row_ptr)
+    rows = np.concatenate([np.argsort(-deg)[:3], np.argsort(deg)[:3], [5, 77, 1234]]).astype(np.int32)
+    Fg = h.features(0, rows)
+    h.close()
+    Fr = M.features(ds.csr_row_ptr, ds.csr_col, ds.csr_val, T0, f, lam, rows)
+    np.testing.assert_array_equal(Fg[:, 2 * f + 1:], 0.0)
+    rel = np.abs(Fg[:, :2 * f + 1] - Fr[:, :2 * f + 1]) / np.maximum(np.abs(Fr[:, :2 * f + 1]), 1e-3)
+    assert rel.max() <= 2.0 ** -7, rel.max()
+
+
+def _cg_handle(ds, f, lam, T, **opt):
+    h = G.ALS(f, lam, solver=G.CUMF_SOLVER_CG, split=256, **opt)
+    h.set_dataset(ds)
+    h.set_factors(theta=T)
+    return h
+
+
+def test_learned_start_trains_and_beats_cold_start():
+    """Harvest 3 plain iterations from one random start, train, then on the
+    first X half-sweep of a run from another random start compare the CG
+    start residual ||b - A x0|| / ||b|| of the network against x0 = 0."""
+    ds = _data.netflix_like_small()
+    f, lam = 32, 0.058
+    net = NN.MLP(f, seed=1)
+    h = G.ALS(f, lam)
+    h.set_dataset(ds)
+    h.set_factors(theta=cumf.init_factors(ds.n, f, 11))
+    net.harvest(h, 3)
+    h.close()
+    assert net.harvested() == 3 * ds.m
+    loss0 = net.train(1, batch=1024, lr=1e-3, seed=0)
+    loss = net.train(30, batch=1024, lr=1e-3, seed=1)
+    assert loss < loss0, (loss0, loss)
+    T1 = cumf.init_factors(ds.n, f, 12)
+    with _cg_handle(ds, f, lam, T1, warm_start=G.CUMF_WARM_ZERO, cg_iters=3, cg_rounds=1, cg_tol=1e-2) as hz:
+        tz = hz.cg_trace(0)
+    with _cg_handle(ds, f, lam, T1, warm_start=G.CUMF_WARM_MLP, cg_iters=3, cg_rounds=1, cg_tol=1e-2) as hm:
+        hm.set_mlp(net)
+        tm = hm.cg_trace(0)
+        st = hm.stats()
+    assert np.allclose(np.median(tz[:, 0]), 1.0)
+    med = float(np.median(tm[:, 0]))
+    print(f"loss {loss0:.4g} -> {loss:.4g}; start residual: zero 1.0, network {med:.3f}; fell back: zero {tz[:, 3].mean():.3f}, "
+          f"network {tm[:, 3].mean():.3f}; warm ms {st['warm_ms']:.2f}")
+    assert med < 0.7, med
+    assert tm[:, 3].mean() <= tz[:, 3].mean()
+
+
+def test_poisoned_network_falls_back_to_cholesky():
+    ds = _data.netflix_like_small()
+    f, lam = 32, 0.058
+    T0 = cumf.init_factors(ds.n, f, 13)
+    with G.ALS(f, lam, split=256) as h:
+        h.set_dataset(ds)
+        h.set_factors(theta=T0)
+        h.update_x()
+        Xc, _ = h.get_factors()
+    with NN.MLP(f, seed=2) as net:
+        P = net.get_params()
+        P[5][:] = np.nan
+        net.set_params(P)
+        with _cg_handle(ds, f, lam, T0, warm_start=G.CUMF_WARM_MLP) as hm:
+            hm.set_mlp(net)
+            hm.update_x()
+            Xg, _ = hm.get_factors()
+            st = hm.stats()
+    assert st["cg_fallback_rows"] == int((np.diff(ds.csr_row_ptr) > 0).sum())
+    rel = np.linalg.norm(Xg - Xc, axis=1) / np.maximum(np.linalg.norm(Xc, axis=1), 1e-30)
+    assert rel.max() <= 1e-6, rel.max()
+
+
+def test_best_start_and_graph_replay():
+    """CUMF_WARM_BEST under hipGraph replay gives the eager factors, and its
+    start residual is never above the current factor's."""
+    ds = _data.netflix_like_small()
+    f, lam = 32, 0.058
+    net = NN.MLP(f, seed=5)
+    T0 = cumf.init_factors(ds.n, f, 14)
+    out = []
+    for graph in (0, 1):
+        with _cg_handle(ds, f, lam, T0, warm_start=G.CUMF_WARM_BEST, use_graph=graph) as h:
+            h.set_mlp(net)
+            h.iterate(2)
+            out.append(h.get_factors())
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    X1, T1 = out[0]
+    with _cg_handle(ds, f, lam, T1, warm_start=G.CUMF_WARM_PREV) as hp:
+        hp.set_factors(X=X1, theta=T1)
+        tp = hp.cg_trace(0)
+    with _cg_handle(ds, f, lam, T1, warm_start=G.CUMF_WARM_BEST) as hb:
+        hb.set_mlp(net)
+        hb.set_factors(X=X1, theta=T1)
+        tb = hb.cg_trace(0)
+    ok = np.isfinite(tp[:, 0]) & np.isfinite(tb[:, 0])
+    assert ok.sum() >= 0.99 * ds.m
+    assert np.all(tb[ok, 0] <= tp[ok, 0] * (1 + 1e-6) + 1e-12)
+    net.close()
