@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""ALS benchmark on one MI355X: seconds per iteration at f=100.
+"""ALS benchmark on one MI355X: seconds per iteration at f=100, against the
+plain FP32 path at equal held-out RMSE.
 
 Workload (default): the Netflix-shaped synthetic rating matrix
 m=480,189 users x n=17,770 items, 99,072,112 training ratings (power-law
@@ -7,6 +8,12 @@ degrees; rows Zipf(1.0) capped at 17,000, columns Zipf(0.8)), a 1% held-out
 set, f=100, lambda=0.058.  One step = one ALS iteration = X half-sweep
 (users from items over the CSR) + Theta half-sweep (items from users over
 the CSC), with ratings and factors already resident in device memory.
+
+Default path: Gram on BF16 MFMA (FP32 accumulation), FP16 conjugate
+gradients with FP32 refinement and an FP64 residual, Cholesky fallback,
+each row warm-started from its current factor.  The plain FP32 path (FP32
+Gram + FP32 Cholesky) runs the same iterations from the same start in the
+same process for the comparison; --no-plain skips it.
 
 Prints ONE JSON line: the metric, throughput figures (s/iteration,
 ratings/s, hermitian TFLOP/s), the roofline of the dominant kernel
@@ -43,7 +50,6 @@ def cpu_baseline(ds, f, lam, theta, X, target_s):
     from oracle import oracle as O
 
     threads = O.max_threads()
-    out = {}
     total = 0.0
     sample_desc = []
     for side, (rp, ci, cv, src) in {"x": (ds.csr_row_ptr, ds.csr_col, ds.csr_val, theta),
@@ -65,13 +71,68 @@ def cpu_baseline(ds, f, lam, theta, X, target_s):
             if dt > 0.3 * target_s / 2 or stride == 1:
                 break
             stride = max(1, int(stride * dt / (target_s / 2)))
-        out[side] = rate
         total += ds.nnz / rate
         sample_desc.append(f"{side}: every {stride}th row ({sel.shape[0]} rows, {int(sub_rp[-1])} ratings, {dt:.1f}s)")
     return {"value": total, "unit": "s/iteration", "cores": threads, "kind": "port",
             "sample": "FP64 normal equations + Cholesky on " + "; ".join(sample_desc)
                       + "; scaled to all 2 x 99,072,112 ratings of one iteration",
             "ratings_per_s": ds.nnz / total}
+
+
+def train_network(G, ds, f, lam, args, prec):
+    """Warm-start network trained on pairs the plain solver produces from
+    another random start on the same ratings (not timed)."""
+    import cumf
+    from cumf import mlp as NN
+    net = NN.MLP(f, seed=1)
+    hh = G.ALS(f, lam, device=0, split=args.split, precision=prec)
+    hh.set_dataset(ds)
+    hh.set_factors(theta=cumf.init_factors(ds.n, f, 1))
+    t1 = time.perf_counter()
+    net.harvest(hh, args.mlp_harvest)
+    t_h = time.perf_counter() - t1
+    hh.close()
+    t1 = time.perf_counter()
+    loss = net.train(args.mlp_epochs, batch=args.mlp_batch, lr=1e-3, seed=0)
+    t_t = time.perf_counter() - t1
+    learned = {"harvest_iterations": args.mlp_harvest, "pairs": net.harvested(), "harvest_s": t_h,
+               "epochs": args.mlp_epochs, "batch": args.mlp_batch, "train_s": t_t, "final_loss": loss}
+    log(f"warm-start network: {learned}")
+    return net, learned
+
+
+def run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net=None):
+    """W untimed + K timed iterations of one path; returns its figures."""
+    h = G.ALS(f, lam, device=0, timing=1, split=args.split, precision=prec, use_graph=int(args.graph),
+              solver=solver, cg_iters=args.cg_iters, cg_rounds=args.cg_rounds, cg_tol=args.cg_tol,
+              warm_start=warm)
+    if net is not None:
+        h.set_mlp(net)
+    h.set_dataset(ds)
+    h.set_factors(theta=theta0)
+    for _ in range(args.warmup):
+        h.iterate(1)
+    h.synchronize()
+    h.reset_stats()
+    t0 = time.perf_counter()
+    h.iterate(args.steps)
+    h.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = h.stats()
+    rmse = h.rmse()
+    X, T = h.get_factors()
+    # Per-half-sweep split, after the timed region (one more iteration).
+    sweep_ms = {}
+    for name, fn in (("x", h.update_x), ("theta", h.update_theta)):
+        h.synchronize()
+        t1 = time.perf_counter()
+        fn()
+        h.synchronize()
+        sweep_ms[name] = (time.perf_counter() - t1) * 1e3
+    opt = h.opt
+    h.close()
+    return {"s": elapsed / args.steps, "rmse": rmse, "stats": st, "sweep_ms": sweep_ms, "X": X, "T": T,
+            "opt": opt}
 
 
 def main():
@@ -83,12 +144,18 @@ def main():
     ap.add_argument("--f", type=int, default=100)
     ap.add_argument("--lam", type=float, default=0.058)
     ap.add_argument("--split", type=int, default=0)
-    ap.add_argument("--precision", choices=["fp32", "bf16gram"], default="fp32")
-    ap.add_argument("--solver", choices=["cholesky", "cg"], default="cholesky")
+    ap.add_argument("--precision", choices=["fp32", "bf16gram"], default="bf16gram")
+    ap.add_argument("--solver", choices=["cholesky", "cg"], default="cg")
     ap.add_argument("--cg-iters", type=int, default=0)
     ap.add_argument("--cg-rounds", type=int, default=0)
     ap.add_argument("--cg-tol", type=float, default=0.0)
+    ap.add_argument("--warm", choices=["prev", "zero", "mlp", "best"], default="prev",
+                    help="CG start; mlp/best train the warm-start network first (untimed)")
+    ap.add_argument("--mlp-harvest", type=int, default=3, help="plain iterations harvested for training")
+    ap.add_argument("--mlp-epochs", type=int, default=4)
+    ap.add_argument("--mlp-batch", type=int, default=4096)
     ap.add_argument("--graph", action="store_true", help="replay each iteration as a captured hipGraph")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -116,36 +183,25 @@ def main():
     theta0 = cumf.init_factors(ds.n, f, 0)
     prec = {"fp32": G.CUMF_PREC_FP32, "bf16gram": G.CUMF_PREC_BF16_GRAM}[args.precision]
     solver = {"cholesky": G.CUMF_SOLVER_CHOLESKY, "cg": G.CUMF_SOLVER_CG}[args.solver]
-    h = G.ALS(f, lam, device=0, timing=1, split=args.split, precision=prec, use_graph=int(args.graph),
-              solver=solver, cg_iters=args.cg_iters, cg_rounds=args.cg_rounds, cg_tol=args.cg_tol)
-    h.set_dataset(ds)
-    h.set_factors(theta=theta0)
-    log("ratings and factors resident on device")
-    for _ in range(args.warmup):
-        h.iterate(1)
-    h.synchronize()
-    h.reset_stats()
-    log(f"warmup {args.warmup} iterations done")
-    t0 = time.perf_counter()
-    h.iterate(args.steps)
-    h.synchronize()
-    elapsed = time.perf_counter() - t0
-    st = h.stats()
-    rmse = h.rmse()
-    X, T = h.get_factors()
-    log(f"timed {args.steps} iterations: {elapsed:.3f}s; rmse={rmse:.5f}; stats={st}")
-    # Per-half-sweep split, after the timed region (one more iteration).
-    sweep_ms = {}
-    for name, fn in (("x", h.update_x), ("theta", h.update_theta)):
-        h.synchronize()
-        t1 = time.perf_counter()
-        fn()
-        h.synchronize()
-        sweep_ms[name] = (time.perf_counter() - t1) * 1e3
-    opt = h.opt
-    h.close()
+    warm = {"prev": G.CUMF_WARM_PREV, "zero": G.CUMF_WARM_ZERO, "mlp": G.CUMF_WARM_MLP,
+            "best": G.CUMF_WARM_BEST}[args.warm]
+    if solver != G.CUMF_SOLVER_CG:
+        warm = G.CUMF_WARM_PREV
+    learned, net = None, None
+    if warm in (G.CUMF_WARM_MLP, G.CUMF_WARM_BEST):
+        net, learned = train_network(G, ds, f, lam, args, prec)
 
-    s_per_iter = elapsed / args.steps
+    res = run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net)
+    st, opt = res["stats"], res["opt"]
+    log(f"timed {args.steps} iterations: {res['s']:.4f} s/iteration; rmse={res['rmse']:.5f}; stats={st}")
+    is_plain = prec == G.CUMF_PREC_FP32 and solver == G.CUMF_SOLVER_CHOLESKY
+    plain = None
+    if not is_plain and not args.no_plain:
+        plain = run_path(G, ds, theta0, f, lam, args, G.CUMF_PREC_FP32, G.CUMF_SOLVER_CHOLESKY,
+                         G.CUMF_WARM_PREV)
+        log(f"plain FP32: {plain['s']:.4f} s/iteration; rmse={plain['rmse']:.5f}")
+
+    s_per_iter = res["s"]
     gram_flops_iter = 2.0 * ds.nnz * f * (f + 1) * 2
     if st["task_launches"]:  # per-kernel events
         kern_ms_total, launches, kname = st["task_ms"], st["task_launches"], "als_task_kernel (fused gram+solve)"
@@ -154,7 +210,15 @@ def main():
     task_ms = kern_ms_total / max(launches, 1)
     gram_flops_launch = st["gram_flops"] / max(launches, 1)
     achieved = gram_flops_launch / (task_ms * 1e-3) / 1e12
-    peak = PEAK_FP32_MFMA_TFLOPS if args.precision == "fp32" else PEAK_BF16_MFMA_TFLOPS
+    peak = PEAK_FP32_MFMA_TFLOPS if prec == G.CUMF_PREC_FP32 else PEAK_BF16_MFMA_TFLOPS
+    factor_bytes = 4 * f if prec == G.CUMF_PREC_FP32 else 2 * ((f + 7) // 8 * 8)
+    gather_bytes_launch = ds.nnz * (8 + factor_bytes)  # col + rating + one factor row per rating
+    if solver == G.CUMF_SOLVER_CHOLESKY:
+        path = ("fp32" if prec == G.CUMF_PREC_FP32 else "bf16") + " gram (MFMA) + fp32 cholesky"
+    else:
+        path = (("fp32" if prec == G.CUMF_PREC_FP32 else "bf16") + " gram (MFMA, fp32 accumulate) + "
+                f"fp16 cg ({opt.cg_iters} steps x {opt.cg_rounds} fp32 refinement rounds, fp64 residual, "
+                f"tol {opt.cg_tol:g}, cholesky fallback, start {args.warm})")
     rec = {
         "metric": METRIC,
         "value": s_per_iter,
@@ -166,34 +230,42 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp32" if args.precision == "fp32" else "bf16 gram")
-                 + (" / fp32 solve" if args.solver == "cholesky" else " / fp16 cg + fp32 refinement + fp64 residual"),
+        "dtype": ("fp32" if prec == G.CUMF_PREC_FP32 else "bf16 gram")
+                 + (" / fp32 solve" if solver == G.CUMF_SOLVER_CHOLESKY
+                    else " / fp16 cg + fp32 refinement + fp64 residual"),
         "data": "synthetic",
         "config": {"workload": args.workload, "m": ds.m, "n": ds.n, "nnz": ds.nnz, "f": f,
-                   "lambda": lam, "precision": {"fp32": "fp32 gram (MFMA) + fp32 cholesky",
-                                                "bf16gram": "bf16 gram (MFMA, fp32 accumulate) + fp32 cholesky"}[args.precision],
-                   "solver": args.solver if args.solver == "cholesky" else
-                   f"cg (fp16, {opt.cg_iters} steps x {opt.cg_rounds} fp32 refinement rounds, fp64 residual, "
-                   f"tol {opt.cg_tol:g}, cholesky fallback)",
-                   "graph": bool(args.graph),
-                   "split": args.split or 2048},
+                   "lambda": lam, "path": path, "graph": bool(args.graph), "split": args.split or 2048,
+                   "iterations_timed": f"{args.warmup + 1}..{args.warmup + args.steps}"},
         "ratings_per_s": ds.nnz / s_per_iter,
         "hermitian_tflops": gram_flops_iter / (kern_ms_total / args.steps * 1e-3) / 1e12,
-        "heldout_rmse": rmse,
+        "heldout_rmse": res["rmse"],
         "failed_rows": st["failed_rows"],
         "cg_fallback_rows_per_iter": st["cg_fallback_rows"] / args.steps,
-        "half_sweep_ms": sweep_ms,
         "kernel_ms_per_iter": {"als_task_kernel": st["task_ms"] / args.steps,
-                               "als_finish_kernel": st["finish_ms"] / args.steps},
+                               "als_finish_kernel": st["finish_ms"] / args.steps,
+                               "als_fallback_kernel": st["fallback_ms"] / args.steps,
+                               "warm_start": st["warm_ms"] / args.steps},
+        "half_sweep_ms": res["sweep_ms"],
+        "learned": learned,
         "roofline": {"kernel": kname, "bound": "mfma",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "avg_launch_ms": task_ms,
                      "flops_per_launch": gram_flops_launch},
+        "gather_roofline": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
+                            "bytes_per_launch": gather_bytes_launch,
+                            "achieved": gather_bytes_launch / (task_ms * 1e-3) / 1e9,
+                            "frac": gather_bytes_launch / (task_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
     }
+    if plain is not None:
+        rec["plain_fp32"] = {"s_per_iteration": plain["s"], "heldout_rmse": plain["rmse"],
+                             "half_sweep_ms": plain["sweep_ms"]}
+        rec["speedup_vs_plain_fp32"] = plain["s"] / s_per_iter
+        rec["rmse_delta_vs_plain_fp32"] = res["rmse"] - plain["rmse"]
     if not args.no_cpu_baseline:
         log("timing CPU baseline")
-        rec["cpu_baseline"] = cpu_baseline(ds, f, lam, theta0, X, args.cpu_seconds)
+        rec["cpu_baseline"] = cpu_baseline(ds, f, lam, theta0, res["X"], args.cpu_seconds)
     print(json.dumps(rec), flush=True)
 
 

@@ -93,7 +93,7 @@ int check_f(int f) {
 
 // Kernel-time bookkeeping with HIP events.
 struct Timer {
-  enum Kind { TASK = 0, FINISH, RMSE, SWEEP, ITER, WARM, NKINDS };
+  enum Kind { TASK = 0, FINISH, RMSE, SWEEP, ITER, WARM, FALLBACK, NKINDS };
   struct Pending { hipEvent_t a, b; int kind; };
   std::vector<hipEvent_t> pool;
   std::vector<Pending> pending;
@@ -148,6 +148,8 @@ struct cumf_handle {
   // X side's rows.
   cumf_mlp* mlp = nullptr;
   DevBuf feat, x0;
+  // Rows the CG pass hands to the Cholesky fallback, and their count.
+  DevBuf fb_list, fb_count;
   // One captured iteration (use_graph), rebuilt after cumf_set_ratings.
   hipGraphExec_t iter_graph = nullptr;
   bool capturing = false;
@@ -258,6 +260,20 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
   a.cg_tol = h->opt.cg_tol;
   a.cg_fallback = h->fail.as<int>() + 1;
   a.cg_trace = trace;
+  const bool cg = h->opt.solver == CUMF_SOLVER_CG;
+  if (cg) {
+    const size_t need = sizeof(int) * (size_t)std::max<int64_t>(p.rows, 1);
+    if (h->fb_list.bytes < need) {
+      if (h->capturing) {
+        cumf::set_last_error("fallback list not allocated before capture");
+        return CUMF_ERR_STATE;
+      }
+      CUMF_TRY(h->fb_list.alloc(need));
+    }
+    if (!h->fb_count.p) CUMF_TRY(h->fb_count.alloc(sizeof(int)));
+    a.fb_list = h->fb_list.as<int>();
+    a.fb_count = h->fb_count.as<int>();
+  }
   a.warm = h->opt.warm_start;
   if (side == 1 && a.warm >= CUMF_WARM_MLP) a.warm = CUMF_WARM_PREV;  // the network predicts X rows
   if (side == 0 && uses_network(h)) {
@@ -281,9 +297,11 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
     s1 = h->timer.get();
     if (s0) (void)hipEventRecord(s0, h->stream);
   }
+  if (cg) CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, h->stream));
   CUMF_TRY(timed(h, Timer::TASK, [&] { return cumf::launch_task_kernel(a, h->stream); }));
   if (p.nheavy > 0)
     CUMF_TRY(timed(h, Timer::FINISH, [&] { return cumf::launch_finish_kernel(a, h->stream); }));
+  if (cg) CUMF_TRY(timed(h, Timer::FALLBACK, [&] { return cumf::launch_fallback_kernel(a, h->stream); }));
   if (s0 && s1) {
     (void)hipEventRecord(s1, h->stream);
     h->timer.pending.push_back({s0, s1, Timer::SWEEP});
@@ -590,6 +608,11 @@ namespace {
 int capture_iteration(cumf_handle* h) {
   h->drop_graph();
   CUMF_TRY(prepare_network(h));
+  if (h->opt.solver == CUMF_SOLVER_CG) {
+    const size_t need = sizeof(int) * (size_t)std::max<int64_t>(std::max(h->m, h->n), 1);
+    if (h->fb_list.bytes < need) CUMF_TRY(h->fb_list.alloc(need));
+    if (!h->fb_count.p) CUMF_TRY(h->fb_count.alloc(sizeof(int)));
+  }
   hipGraph_t g = nullptr;
   CUMF_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   h->capturing = true;
@@ -693,6 +716,8 @@ int cumf_get_stats(cumf_handle* h, cumf_stats* s) {
   s->cg_fallback_rows = fails[1];
   s->warm_ms = h->timer.ms[Timer::WARM];
   s->warm_launches = h->timer.count[Timer::WARM];
+  s->fallback_ms = h->timer.ms[Timer::FALLBACK];
+  s->fallback_launches = h->timer.count[Timer::FALLBACK];
   s->gram_flops = h->gram_flops;
   s->iter_ms = h->timer.ms[Timer::ITER];
   s->iters = h->timer.count[Timer::ITER];

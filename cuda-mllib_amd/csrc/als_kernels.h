@@ -60,6 +60,10 @@ struct SweepArgs {
   int warm;
   const float* x0;
   float* cg_trace;         // optional, 4 floats per row: initial / final relative residual, rounds, fell back
+  // Rows the CG pass hands to the Cholesky fallback kernel: entry = row, or
+  // -1 - index into heavy for a split row; fb_count = list length.
+  int* fb_list;
+  int* fb_count;
 };
 
 // The reference-reproducing path (compat.hip): one work-group per row.
@@ -84,6 +88,8 @@ size_t partial_floats_per_slot(int f);
 int dump_dim(int f);
 hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s);
 hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s);
+hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s);
+hipError_t launch_fallback_reset(int* count, hipStream_t s);
 hipError_t launch_dump_kernel(const SweepArgs& a, const int32_t* rows, int64_t nrows, float* out,
                               hipStream_t s);
 hipError_t launch_rmse(const int32_t* u, const int32_t* v, const float* r, int64_t cnt, const float* X,

@@ -504,6 +504,14 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return readlane_d(v, 0);
 }
 
+// A register value the compiler must treat as changed at this point: keeps
+// loop-invariant FP64 conversions of the Gram from being hoisted out of the
+// refinement loop (which would hold 2x the Gram's registers live).
+__device__ __forceinline__ f4 opaque(f4 v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // ys[0, 16NB) = A xs in FP64, A the Gram in registers (upper tiles; the
 // diagonal tiles are full), xs zero beyond f.  Row direction: lane (g, c)
 // of tile (I, J) adds A[16I+4g+r][16J+c] x[16J+c] to row 16I+4g+r; the 16
@@ -524,7 +532,7 @@ __device__ __forceinline__ void gram_matvec_f64(const f4 (&acc)[num_tiles(NB)], 
     double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
 #pragma unroll
     for (int J = I; J < NB; ++J) {
-      const f4 v = acc[tile_index(I, J, NB)];
+      const f4 v = opaque(acc[tile_index(I, J, NB)]);
       y0 = fma((double)v[0], xc[J], y0);
       y1 = fma((double)v[1], xc[J], y1);
       y2 = fma((double)v[2], xc[J], y2);
@@ -548,7 +556,7 @@ __device__ __forceinline__ void gram_matvec_f64(const f4 (&acc)[num_tiles(NB)], 
 #pragma unroll
     for (int I = 0; I < J; ++I) {
       const f4 xr = *reinterpret_cast<const f4*>(xs + 16 * I + 4 * g);
-      const f4 v = acc[tile_index(I, J, NB)];
+      const f4 v = opaque(acc[tile_index(I, J, NB)]);
 #pragma unroll
       for (int r = 0; r < 4; ++r) yc = fma((double)v[r], (double)xr[r], yc);
     }
@@ -613,15 +621,15 @@ __device__ __forceinline__ double residual_norm2(const f4 (&acc)[num_tiles(NB)],
   return wave_sum_d(rr);
 }
 
-// Returns 1 if x was accepted, 0 if the Cholesky fallback ran (ok reports
-// the fallback's pivots), and writes x to xrow / xb as chol_solve does.
+// Returns true (and writes x to xrow / xb) if x was accepted; false leaves
+// the row to the Cholesky fallback kernel.
 // x0row: the network's prediction for this row (warm 2 and 3).  trace
 // (optional): initial and final relative residual, rounds run, fell back.
 template <int NB>
 __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bcol, float* __restrict__ lds,
                                          float* __restrict__ xrow, int64_t ldf, int lane, int bcol2,
                                          __bf16* __restrict__ xb, int64_t ldb, const CgParams& cp,
-                                         const float* __restrict__ x0row, float* __restrict__ trace, bool& ok) {
+                                         const float* __restrict__ x0row, float* __restrict__ trace) {
   constexpr int RPL = rows_per_lane(NB);
   constexpr int D = 16 * NB;
   const int g = lane >> 4, c = lane & 15;
@@ -815,11 +823,7 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
     trace[2] = (float)rounds_run;
     trace[3] = accepted ? 0.0f : 1.0f;
   }
-  if (!accepted) {
-    lds_order();
-    ok = chol_solve<NB>(acc, f, bcol, lds, xrow, ldf, lane, bcol2, xb, ldb);
-    return false;
-  }
+  if (!accepted) return false;
 #pragma unroll
   for (int t = 0; t < RPL; ++t) {
     const int i = lane + 64 * t;
@@ -827,7 +831,6 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
     if (i < ldf) xrow[i] = xo;
     if (xb && i < ldb) xb[i] = (__bf16)xo;
   }
-  ok = true;
   return true;
 }
 
@@ -845,26 +848,27 @@ __device__ __forceinline__ void gram_any(f4 (&acc)[num_tiles(NB)], const SweepAr
 }
 
 // Solves the system in registers for one row: Cholesky, or (CG) the
-// mixed-precision refinement with its Cholesky fallback.  Counts rows whose
-// factorisation failed and rows that fell back.
+// mixed-precision refinement; a row CG does not accept is appended to the
+// fallback list as `entry` (row >= 0, or -1 - index of a split row).
 template <int NB, bool CG>
 __device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepArgs& a, bool bf, float* lds,
-                                          int64_t row, __bf16* xb, int lane) {
+                                          int64_t row, __bf16* xb, int lane, int entry) {
   float* xrow = a.x + row * a.ldf;
   const int bcol = bf ? a.f : (int)a.ldf;
   const int bcol2 = bf ? a.f + 1 : -1;
-  bool ok;
   if constexpr (CG) {
     const CgParams cp{a.cg_iters, a.cg_rounds, a.cg_tol, a.warm};
     const float* x0row = a.x0 ? a.x0 + row * a.ldf : nullptr;
     float* trace = a.cg_trace ? a.cg_trace + 4 * row : nullptr;
-    const bool accepted =
-        cg_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb, cp, x0row, trace, ok);
-    if (!accepted && lane == 0) atomicAdd(a.cg_fallback, 1);
+    const bool accepted = cg_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb, cp, x0row, trace);
+    if (!accepted && lane == 0) {
+      a.fb_list[atomicAdd(a.fb_count, 1)] = entry;
+      atomicAdd(a.cg_fallback, 1);
+    }
   } else {
-    ok = chol_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb);
+    const bool ok = chol_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb);
+    if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
   }
-  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
 }
 
 template <int NB, bool BF, bool CG>
@@ -897,7 +901,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
   }
   add_ridge<NB>(acc, a.f, a.lambda * (float)t.count, lane);
   lds_order();
-  solve_row<NB, CG>(acc, a, BF, lds, t.row, xb, lane);
+  solve_row<NB, CG>(acc, a, BF, lds, t.row, xb, lane, t.row);
 }
 
 template <int NB, bool CG>
@@ -920,8 +924,50 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
   const bool bf = a.precision != 0;
   __bf16* xb = bf ? reinterpret_cast<__bf16*>(a.x_b) + (int64_t)h.row * a.ldb : nullptr;
   lds_order();
-  solve_row<NB, CG>(acc, a, bf, lds, h.row, xb, lane);
+  solve_row<NB, CG>(acc, a, bf, lds, h.row, xb, lane, -1 - (int)blockIdx.x);
 }
+
+// Cholesky solve of the rows the CG pass did not accept: a light row's Gram
+// is formed again from its ratings, a split row's from its partial Grams
+// (still in the scratch buffer).  A fixed grid walks the list, whose length
+// the CG pass left in fb_count.
+template <int NB, bool BF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_fallback_kernel(SweepArgs a) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const int total = *a.fb_count;
+  for (int k = blockIdx.x; k < total; k += gridDim.x) {
+    const int e = a.fb_list[k];
+    f4 acc[num_tiles(NB)];
+#pragma unroll
+    for (int i = 0; i < num_tiles(NB); ++i) acc[i] = f4{0.f, 0.f, 0.f, 0.f};
+    int64_t row;
+    if (e >= 0) {
+      row = e;
+      const int64_t s = a.row_ptr[row];
+      const int cnt = a.row_ptr[row + 1] - a.row_ptr[row];
+      lds_order();
+      if (cnt > 0) gram_any<NB, BF>(acc, a, s, cnt, lds, lane);
+    } else {
+      const HeavyRow h = a.heavy[-1 - e];
+      row = h.row;
+      for (int p = 0; p < h.nchunks; ++p) {
+        const float* src = a.partial + (int64_t)(h.slot + p) * num_tiles(NB) * 256;
+#pragma unroll
+        for (int i = 0; i < num_tiles(NB); ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][r] += src[(i * 4 + r) * 64 + lane];
+      }
+    }
+    const int n_u = a.row_ptr[row + 1] - a.row_ptr[row];
+    add_ridge<NB>(acc, a.f, a.lambda * (float)n_u, lane);
+    __bf16* xb = BF ? reinterpret_cast<__bf16*>(a.x_b) + row * a.ldb : nullptr;
+    lds_order();
+    solve_row<NB, false>(acc, a, BF, lds, row, xb, lane, 0);
+  }
+}
+
+__global__ void fb_reset_kernel(int* count) { *count = 0; }
 
 // Gram dump for tests: the full augmented (16NB)^2 matrix of each listed
 // row (ridge included), row-major, upper tiles mirrored.
@@ -1058,6 +1104,8 @@ int sweep_nb(int f) {
   return (w + 15) / 16;
 }
 
+constexpr int kFallbackGrid = 2048;  // work-groups walking the fallback list
+
 template <int NB>
 static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which, const int32_t* rows,
                             float* out, int64_t nrows) {
@@ -1078,6 +1126,10 @@ static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which, const 
       if (cg) hipLaunchKernelGGL((als_finish_kernel<NB, true>), grid, dim3(64), shmem, s, a);
       else hipLaunchKernelGGL((als_finish_kernel<NB, false>), grid, dim3(64), shmem, s, a);
     }
+  } else if (which == 3) {
+    const size_t shc = sizeof(float) * (size_t)lds_floats_nb(NB, a.f);
+    if (bf) hipLaunchKernelGGL((als_fallback_kernel<NB, true>), dim3(kFallbackGrid), dim3(64), shc, s, a);
+    else hipLaunchKernelGGL((als_fallback_kernel<NB, false>), dim3(kFallbackGrid), dim3(64), shc, s, a);
   } else {
     if (nrows > 0) {
       if (bf) hipLaunchKernelGGL((als_dump_kernel<NB, true>), dim3((unsigned)nrows), dim3(64), shmem, s, a, rows, out);
@@ -1104,6 +1156,11 @@ static hipError_t dispatch(const SweepArgs& a, hipStream_t s, int which, const i
 }
 
 hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 0); }
+hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 3); }
+hipError_t launch_fallback_reset(int* count, hipStream_t s) {
+  hipLaunchKernelGGL(fb_reset_kernel, dim3(1), dim3(1), 0, s, count);
+  return hipGetLastError();
+}
 hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 1); }
 hipError_t launch_dump_kernel(const SweepArgs& a, const int32_t* rows, int64_t nrows, float* out,
                               hipStream_t s) {

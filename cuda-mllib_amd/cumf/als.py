@@ -42,7 +42,8 @@ class Stats(C.Structure):
                 ("sweep_ms", C.c_double), ("sweeps", C.c_int64),
                 ("failed_rows", C.c_int64), ("gram_flops", C.c_int64),
                 ("iter_ms", C.c_double), ("iters", C.c_int64),
-                ("cg_fallback_rows", C.c_int64), ("warm_ms", C.c_double), ("warm_launches", C.c_int64)]
+                ("cg_fallback_rows", C.c_int64), ("warm_ms", C.c_double), ("warm_launches", C.c_int64),
+                ("fallback_ms", C.c_double), ("fallback_launches", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -129,6 +129,8 @@ typedef struct cumf_stats {
   int64_t cg_fallback_rows; /* CUMF_SOLVER_CG: rows solved again by Cholesky */
   double warm_ms;          /* features + network predictions (CUMF_WARM_MLP/BEST) */
   int64_t warm_launches;
+  double fallback_ms;      /* Cholesky re-solve of the rows CG did not accept */
+  int64_t fallback_launches;
 } cumf_stats;
 
 typedef struct cumf_handle cumf_handle;
