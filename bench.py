@@ -79,6 +79,23 @@ def cpu_baseline(ds, f, lam, theta, X, target_s):
             "ratings_per_s": ds.nnz / total}
 
 
+def measured_traffic(workload, precision, solver, f):
+    """HBM bytes per launch of the fused kernel for this workload and path,
+    from the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE,
+    each scaled by what the counter reads for a copy of known size); None if
+    that configuration was not profiled."""
+    path = os.path.join(REPO, "profiles", "traffic_per_launch.txt")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 5 and not line.startswith("#") and parts[:4] == [workload, precision, solver, str(f)]:
+                    return float(parts[4])
+    except OSError:
+        pass
+    return None
+
+
 def train_network(G, ds, f, lam, args, prec):
     """Warm-start network trained on pairs the plain solver produces from
     another random start on the same ratings (not timed)."""
@@ -202,12 +219,17 @@ def main():
         log(f"plain FP32: {plain['s']:.4f} s/iteration; rmse={plain['rmse']:.5f}")
 
     s_per_iter = res["s"]
-    gram_flops_iter = 2.0 * ds.nnz * f * (f + 1) * 2
+    # Unique symmetric Gram work: f(f+1)/2 entries, one multiply-add each, per
+    # rating and half-sweep.
+    gram_flops_iter = 2.0 * ds.nnz * f * (f + 1)
     if st["task_launches"]:  # per-kernel events
         kern_ms_total, launches, kname = st["task_ms"], st["task_launches"], "als_task_kernel (fused gram+solve)"
     else:  # graph replays: whole iterations
         kern_ms_total, launches, kname = st["iter_ms"], st["iters"], "one captured iteration (hipGraph)"
     task_ms = kern_ms_total / max(launches, 1)
+    # all kernels of the sweeps (fused Gram + solve, split-row finish, fallback)
+    sweep_kernel_ms = (st["task_ms"] + st["finish_ms"] + st["fallback_ms"]) if st["task_launches"] else kern_ms_total
+    traffic = measured_traffic(args.workload, args.precision, args.solver, f) if st["task_launches"] else None
     gram_flops_launch = st["gram_flops"] / max(launches, 1)
     achieved = gram_flops_launch / (task_ms * 1e-3) / 1e12
     peak = PEAK_FP32_MFMA_TFLOPS if prec == G.CUMF_PREC_FP32 else PEAK_BF16_MFMA_TFLOPS
@@ -238,7 +260,7 @@ def main():
                    "lambda": lam, "path": path, "graph": bool(args.graph), "split": args.split or 2048,
                    "iterations_timed": f"{args.warmup + 1}..{args.warmup + args.steps}"},
         "ratings_per_s": ds.nnz / s_per_iter,
-        "hermitian_tflops": gram_flops_iter / (kern_ms_total / args.steps * 1e-3) / 1e12,
+        "hermitian_tflops": gram_flops_iter / (sweep_kernel_ms / args.steps * 1e-3) / 1e12,
         "heldout_rmse": res["rmse"],
         "failed_rows": st["failed_rows"],
         "cg_fallback_rows_per_iter": st["cg_fallback_rows"] / args.steps,
@@ -250,7 +272,7 @@ def main():
         "learned": learned,
         "roofline": {"kernel": kname, "bound": "mfma",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
                      "avg_launch_ms": task_ms,
                      "flops_per_launch": gram_flops_launch},
         "gather_roofline": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,

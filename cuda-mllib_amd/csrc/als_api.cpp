@@ -231,7 +231,7 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
     c.f = h->f;
     c.lambda = h->lambda;
     CUMF_TRY(timed(h, Timer::TASK, [&] { return cumf::launch_compat(c, p.rows, h->stream); }));
-    h->gram_flops += 2 * p.nnz * (int64_t)h->f * (h->f + 1);
+    h->gram_flops += p.nnz * (int64_t)h->f * (h->f + 1);
     return CUMF_OK;
   }
   cumf::SweepArgs a{};
@@ -306,7 +306,7 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
     (void)hipEventRecord(s1, h->stream);
     h->timer.pending.push_back({s0, s1, Timer::SWEEP});
   }
-  h->gram_flops += 2 * p.nnz * (int64_t)h->f * (h->f + 1);
+  h->gram_flops += p.nnz * (int64_t)h->f * (h->f + 1);
   return CUMF_OK;
 }
 
@@ -660,7 +660,7 @@ int cumf_iterate(cumf_handle* h, int iters) {
   if (!h->iter_graph) CUMF_TRY(capture_iteration(h));
   for (int it = 0; it < iters; ++it) {
     CUMF_TRY(timed(h, Timer::ITER, [&] { return hipGraphLaunch(h->iter_graph, h->stream); }));
-    h->gram_flops += 2 * (h->plan_x.nnz + h->plan_t.nnz) * (int64_t)h->f * (h->f + 1);
+    h->gram_flops += (h->plan_x.nnz + h->plan_t.nnz) * (int64_t)h->f * (h->f + 1);
   }
   return CUMF_OK;
 }

@@ -123,7 +123,9 @@ typedef struct cumf_stats {
   double sweep_ms;         /* whole half-sweeps, first to last kernel */
   int64_t sweeps;
   int64_t failed_rows;     /* rows with a non-positive pivot (x_u set to 0) */
-  int64_t gram_flops;      /* 2 nnz f(f+1) summed over the half-sweeps run */
+  int64_t gram_flops;      /* nnz f(f+1) (the f(f+1)/2 unique Gram entries, one
+                              multiply-add each, per rating) summed over the
+                              half-sweeps run */
   double iter_ms;          /* graph replays of whole iterations (use_graph) */
   int64_t iters;
   int64_t cg_fallback_rows; /* CUMF_SOLVER_CG: rows solved again by Cholesky */
