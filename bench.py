@@ -118,7 +118,7 @@ def train_network(G, ds, f, lam, args, prec):
     return net, learned
 
 
-def run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net=None):
+def run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net=None, tune=True):
     """W untimed + K timed iterations of one path; returns its figures."""
     h = G.ALS(f, lam, device=0, timing=1, split=args.split, precision=prec, use_graph=int(args.graph),
               solver=solver, cg_iters=args.cg_iters, cg_rounds=args.cg_rounds, cg_tol=args.cg_tol,
@@ -127,6 +127,17 @@ def run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net=None):
         h.set_mlp(net)
     h.set_dataset(ds)
     h.set_factors(theta=theta0)
+    tuning = None
+    if tune and args.tile_table:
+        h.load_tile_table(args.tile_table)
+        tuning = {"tile_table": args.tile_table}
+    elif tune and args.autotune:
+        out = os.path.join(REPO, "gpurun_out", "tile_table_%s_f%d.txt" % (args.workload, f))
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        t1 = time.perf_counter()
+        h.autotune(reps=2, path=out)
+        tuning = {"autotune_s": time.perf_counter() - t1, "table": open(out).read().splitlines()}
+        log(f"autotuned in {tuning['autotune_s']:.1f}s -> {out}")
     for _ in range(args.warmup):
         h.iterate(1)
     h.synchronize()
@@ -149,7 +160,7 @@ def run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net=None):
     opt = h.opt
     h.close()
     return {"s": elapsed / args.steps, "rmse": rmse, "stats": st, "sweep_ms": sweep_ms, "X": X, "T": T,
-            "opt": opt}
+            "opt": opt, "tuning": tuning}
 
 
 def main():
@@ -171,7 +182,10 @@ def main():
     ap.add_argument("--mlp-harvest", type=int, default=3, help="plain iterations harvested for training")
     ap.add_argument("--mlp-epochs", type=int, default=4)
     ap.add_argument("--mlp-batch", type=int, default=4096)
-    ap.add_argument("--graph", action="store_true", help="replay each iteration as a captured hipGraph")
+    ap.add_argument("--graph", action="store_true", help="replay each half-sweep as a captured hipGraph")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the degree-bucket launch shapes on this workload first (untimed)")
+    ap.add_argument("--tile-table", default="", help="load degree-bucket launch shapes from this file")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -215,17 +229,19 @@ def main():
     plain = None
     if not is_plain and not args.no_plain:
         plain = run_path(G, ds, theta0, f, lam, args, G.CUMF_PREC_FP32, G.CUMF_SOLVER_CHOLESKY,
-                         G.CUMF_WARM_PREV)
+                         G.CUMF_WARM_PREV, tune=False)
         log(f"plain FP32: {plain['s']:.4f} s/iteration; rmse={plain['rmse']:.5f}")
 
     s_per_iter = res["s"]
     # Unique symmetric Gram work: f(f+1)/2 entries, one multiply-add each, per
     # rating and half-sweep.
     gram_flops_iter = 2.0 * ds.nnz * f * (f + 1)
+    # One "launch" = all als_task_kernel launches of one half-sweep (one, or
+    # one per degree bucket with a tile table).
     if st["task_launches"]:  # per-kernel events
-        kern_ms_total, launches, kname = st["task_ms"], st["task_launches"], "als_task_kernel (fused gram+solve)"
-    else:  # graph replays: whole iterations
-        kern_ms_total, launches, kname = st["iter_ms"], st["iters"], "one captured iteration (hipGraph)"
+        kern_ms_total, launches, kname = st["task_ms"], st["sweeps"], "als_task_kernel (fused gram+solve)"
+    else:  # graph replays: whole half-sweeps
+        kern_ms_total, launches, kname = st["iter_ms"], st["iters"], "one captured half-sweep (hipGraph)"
     task_ms = kern_ms_total / max(launches, 1)
     # all kernels of the sweeps (fused Gram + solve, split-row finish, fallback)
     sweep_kernel_ms = (st["task_ms"] + st["finish_ms"] + st["fallback_ms"]) if st["task_launches"] else kern_ms_total
@@ -270,6 +286,7 @@ def main():
                                "warm_start": st["warm_ms"] / args.steps},
         "half_sweep_ms": res["sweep_ms"],
         "learned": learned,
+        "batching": res["tuning"],
         "roofline": {"kernel": kname, "bound": "mfma",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,

@@ -13,6 +13,8 @@
 // order by the finish kernel.  Tasks are ordered by decreasing length so
 // the longest start first (longest-processing-time-first scheduling).
 #include <algorithm>
+#include <climits>
+#include <cstdio>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -40,12 +42,29 @@ constexpr float kDefaultCgTol = 1e-3f;
 
 int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }
 
-// Wavefront task list of one rating matrix side.
+// Degree buckets of the batching form: tasks whose rating count falls in
+// [kBucketEdge[b], kBucketEdge[b + 1]) form bucket b.
+constexpr int kBuckets = 8;
+constexpr int64_t kBucketEdge[kBuckets + 1] = {0, 16, 64, 256, 1024, 4096, 16384, 65536, INT64_MAX};
+
+// Launch shape of one bucket: register budget (wavefronts per SIMD the
+// kernel was compiled for, 1 or 2) and persistent grid (0: one work-group
+// per task; > 0: that many wavefronts taking tasks from a counter).
+struct BucketCfg {
+  int variant = 2;
+  int persistent = 0;
+};
+
+// Wavefront task list of one rating matrix side.  Tasks are ordered by
+// decreasing rating count, so every degree bucket is a contiguous range.
 struct Plan {
   int64_t rows = 0, nnz = 0, ntasks = 0, nheavy = 0, nslots = 0;
+  int split = 0;
+  int64_t bucket_t0[kBuckets] = {}, bucket_t1[kBuckets] = {};
   DevBuf tasks, heavy, partial;
 
-  int build(const int32_t* row_ptr, int64_t m, int split, int f) {
+  int build(const int32_t* row_ptr, int64_t m, int split_, int f) {
+    const int64_t split = split_;
     std::vector<cumf::Task> t;
     std::vector<cumf::HeavyRow> hv;
     t.reserve((size_t)m);
@@ -71,7 +90,15 @@ struct Plan {
                      [](const cumf::Task& a, const cumf::Task& b) { return a.count > b.count; });
     rows = m;
     nnz = row_ptr[m];
+    this->split = (int)split;
     ntasks = (int64_t)t.size();
+    for (int b = 0; b < kBuckets; ++b) bucket_t0[b] = bucket_t1[b] = 0;
+    for (int64_t i = 0; i < ntasks; ++i) {
+      int b = 0;
+      while (b + 1 < kBuckets && t[(size_t)i].count >= kBucketEdge[b + 1]) ++b;
+      if (bucket_t1[b] == 0) bucket_t0[b] = i;
+      bucket_t1[b] = i + 1;
+    }
     nheavy = (int64_t)hv.size();
     nslots = slot;
     CUMF_TRY(tasks.alloc(sizeof(cumf::Task) * t.size()));
@@ -150,12 +177,27 @@ struct cumf_handle {
   DevBuf feat, x0;
   // Rows the CG pass hands to the Cholesky fallback, and their count.
   DevBuf fb_list, fb_count;
-  // One captured iteration (use_graph), rebuilt after cumf_set_ratings.
-  hipGraphExec_t iter_graph = nullptr;
+  // Host copies of the row / column pointers (plans are rebuilt when the
+  // tile table changes the split size).
+  std::vector<int32_t> rp_host, cp_host;
+  // Tile table (batching form): per side, the split size, one launch shape
+  // per degree bucket and the register budget of the finish and fallback
+  // kernels.  Without a table: one launch of all tasks, two-wave budget.
+  bool tiled = false;
+  int tile_split[2] = {0, 0};
+  BucketCfg tiles[2][kBuckets];
+  int finish_variant[2] = {2, 2}, fallback_variant[2] = {2, 2};
+  DevBuf counters;  // persistent task counters, kBuckets per side
+  int num_cus = 0;
+  // One captured half-sweep per side (use_graph), rebuilt after anything
+  // that changes its launches.
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool capturing = false;
   void drop_graph() {
-    if (iter_graph) (void)hipGraphExecDestroy(iter_graph);
-    iter_graph = nullptr;
+    for (auto& g : graph) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
   }
 };
 
@@ -197,6 +239,43 @@ int timed(cumf_handle* h, int kind, F&& launch) {
   return CUMF_OK;
 }
 
+hipError_t task_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
+  return variant == 1 ? cumf::w1::launch_task_kernel(a, s) : cumf::w2::launch_task_kernel(a, s);
+}
+hipError_t finish_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
+  return variant == 1 ? cumf::w1::launch_finish_kernel(a, s) : cumf::w2::launch_finish_kernel(a, s);
+}
+hipError_t fallback_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
+  return variant == 1 ? cumf::w1::launch_fallback_kernel(a, s) : cumf::w2::launch_fallback_kernel(a, s);
+}
+
+// Launch of one degree bucket's tasks with its shape.
+hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs& a, int b,
+                         const BucketCfg& cfg) {
+  cumf::SweepArgs x = a;
+  x.tasks = a.tasks + p.bucket_t0[b];
+  x.ntasks = p.bucket_t1[b] - p.bucket_t0[b];
+  x.persistent = cfg.persistent;
+  x.task_counter = h->counters.as<int>() + side * kBuckets + b;
+  return task_launch(x, cfg.variant, h->stream);
+}
+
+// All tasks of a half-sweep: one launch, or with a tile table one launch
+// per non-empty degree bucket (heaviest first), persistent ones after their
+// counters are cleared.
+int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs& a) {
+  if (!h->tiled) return timed(h, Timer::TASK, [&] { return task_launch(a, 2, h->stream); });
+  bool persistent = false;
+  for (int b = 0; b < kBuckets; ++b) persistent = persistent || h->tiles[side][b].persistent > 0;
+  if (persistent)
+    CUMF_HIP_TRY(cumf::launch_fallback_reset(h->counters.as<int>() + side * kBuckets, kBuckets, h->stream));
+  for (int b = kBuckets - 1; b >= 0; --b) {
+    if (p.bucket_t1[b] <= p.bucket_t0[b]) continue;
+    CUMF_TRY(timed(h, Timer::TASK, [&] { return bucket_launch(h, side, p, a, b, h->tiles[side][b]); }));
+  }
+  return CUMF_OK;
+}
+
 bool uses_network(const cumf_handle* h) {
   return h->opt.solver == CUMF_SOLVER_CG &&
          (h->opt.warm_start == CUMF_WARM_MLP || h->opt.warm_start == CUMF_WARM_BEST);
@@ -216,25 +295,11 @@ int prepare_network(cumf_handle* h) {
   return cumf::mlp_prepare_predict(h->mlp);
 }
 
-int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, const DevBuf& col,
-                   const DevBuf& val, const float* theta, float* x, const DevBuf& theta_b, DevBuf& x_b,
-                   float* trace = nullptr) {
-  if (h->opt.compat_hermitian_bug) {
-    cumf::CompatArgs c{};
-    c.row_ptr = rowp.as<int32_t>();
-    c.col = col.as<int32_t>();
-    c.val = val.as<float>();
-    c.theta = theta;
-    c.x = x;
-    c.ldf = h->ldf;
-    c.fail_counter = h->fail.as<int>();
-    c.f = h->f;
-    c.lambda = h->lambda;
-    CUMF_TRY(timed(h, Timer::TASK, [&] { return cumf::launch_compat(c, p.rows, h->stream); }));
-    h->gram_flops += p.nnz * (int64_t)h->f * (h->f + 1);
-    return CUMF_OK;
-  }
-  cumf::SweepArgs a{};
+// Kernel arguments of a half-sweep over plan p (the CG fallback list is
+// allocated here when missing, which must not happen during a capture).
+int fill_args(cumf_handle* h, const Plan& p, const DevBuf& rowp, const DevBuf& col, const DevBuf& val,
+              const float* theta, float* x, const DevBuf& theta_b, DevBuf& x_b, float* trace,
+              cumf::SweepArgs& a) {
   a.row_ptr = rowp.as<int32_t>();
   a.col = col.as<int32_t>();
   a.val = val.as<float>();
@@ -260,8 +325,7 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
   a.cg_tol = h->opt.cg_tol;
   a.cg_fallback = h->fail.as<int>() + 1;
   a.cg_trace = trace;
-  const bool cg = h->opt.solver == CUMF_SOLVER_CG;
-  if (cg) {
+  if (h->opt.solver == CUMF_SOLVER_CG) {
     const size_t need = sizeof(int) * (size_t)std::max<int64_t>(p.rows, 1);
     if (h->fb_list.bytes < need) {
       if (h->capturing) {
@@ -274,6 +338,30 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
     a.fb_list = h->fb_list.as<int>();
     a.fb_count = h->fb_count.as<int>();
   }
+  return CUMF_OK;
+}
+
+int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, const DevBuf& col,
+                   const DevBuf& val, const float* theta, float* x, const DevBuf& theta_b, DevBuf& x_b,
+                   float* trace = nullptr) {
+  if (h->opt.compat_hermitian_bug) {
+    cumf::CompatArgs c{};
+    c.row_ptr = rowp.as<int32_t>();
+    c.col = col.as<int32_t>();
+    c.val = val.as<float>();
+    c.theta = theta;
+    c.x = x;
+    c.ldf = h->ldf;
+    c.fail_counter = h->fail.as<int>();
+    c.f = h->f;
+    c.lambda = h->lambda;
+    CUMF_TRY(timed(h, Timer::TASK, [&] { return cumf::launch_compat(c, p.rows, h->stream); }));
+    h->gram_flops += p.nnz * (int64_t)h->f * (h->f + 1);
+    return CUMF_OK;
+  }
+  cumf::SweepArgs a{};
+  CUMF_TRY(fill_args(h, p, rowp, col, val, theta, x, theta_b, x_b, trace, a));
+  const bool cg = h->opt.solver == CUMF_SOLVER_CG;
   a.warm = h->opt.warm_start;
   if (side == 1 && a.warm >= CUMF_WARM_MLP) a.warm = CUMF_WARM_PREV;  // the network predicts X rows
   if (side == 0 && uses_network(h)) {
@@ -297,11 +385,12 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
     s1 = h->timer.get();
     if (s0) (void)hipEventRecord(s0, h->stream);
   }
-  if (cg) CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, h->stream));
-  CUMF_TRY(timed(h, Timer::TASK, [&] { return cumf::launch_task_kernel(a, h->stream); }));
+  if (cg) CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
+  CUMF_TRY(launch_tasks(h, side, p, a));
+  const int fv = h->tiled ? h->finish_variant[side] : 2, bv = h->tiled ? h->fallback_variant[side] : 2;
   if (p.nheavy > 0)
-    CUMF_TRY(timed(h, Timer::FINISH, [&] { return cumf::launch_finish_kernel(a, h->stream); }));
-  if (cg) CUMF_TRY(timed(h, Timer::FALLBACK, [&] { return cumf::launch_fallback_kernel(a, h->stream); }));
+    CUMF_TRY(timed(h, Timer::FINISH, [&] { return finish_launch(a, fv, h->stream); }));
+  if (cg) CUMF_TRY(timed(h, Timer::FALLBACK, [&] { return fallback_launch(a, bv, h->stream); }));
   if (s0 && s1) {
     (void)hipEventRecord(s1, h->stream);
     h->timer.pending.push_back({s0, s1, Timer::SWEEP});
@@ -415,6 +504,11 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
     return CUMF_ERR_HIP;
   }
   int rc = h->fail.alloc(2 * sizeof(int));  // failed rows, CG fallbacks
+  if (rc == CUMF_OK) rc = h->counters.alloc(sizeof(int) * 2 * kBuckets);
+  if (rc == CUMF_OK && hipDeviceGetAttribute(&h->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+    (void)hipGetLastError();
+    h->num_cus = 256;
+  }
   if (rc == CUMF_OK) rc = h->rmse_partial.alloc(sizeof(double) * kRmseBlocks);
   if (rc == CUMF_OK) rc = h->rmse_out.alloc(sizeof(double));
   if (rc == CUMF_OK) rc = h->zero16.alloc(64);
@@ -517,8 +611,10 @@ int cumf_set_ratings(cumf_handle* h, int64_t m, int64_t n, int64_t nnz, const in
   }
   CUMF_HIP_TRY(hipMemset(h->X.p, 0, h->X.bytes));
   CUMF_HIP_TRY(hipMemset(h->T.p, 0, h->T.bytes));
-  CUMF_TRY(h->plan_x.build(csr_row_ptr, m, h->opt.split, h->f));
-  CUMF_TRY(h->plan_t.build(csc_col_ptr, n, h->opt.split, h->f));
+  h->rp_host.assign(csr_row_ptr, csr_row_ptr + m + 1);
+  h->cp_host.assign(csc_col_ptr, csc_col_ptr + n + 1);
+  CUMF_TRY(h->plan_x.build(csr_row_ptr, m, h->tile_split[0] > 0 ? h->tile_split[0] : h->opt.split, h->f));
+  CUMF_TRY(h->plan_t.build(csc_col_ptr, n, h->tile_split[1] > 0 ? h->tile_split[1] : h->opt.split, h->f));
   h->m = m;
   h->n = n;
   h->nnz = nnz;
@@ -582,13 +678,17 @@ int cumf_get_factors(cumf_handle* h, float* X, float* theta) {
   return CUMF_OK;
 }
 
+namespace {
+int half_sweep(cumf_handle* h, int side);
+}  // namespace
+
 int cumf_update_x(cumf_handle* h) {
   if (!h || !h->have_ratings) {
     cumf::set_last_error("cumf_update_x: no ratings yet");
     return CUMF_ERR_STATE;
   }
   DeviceGuard dg(h->device);
-  return run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb, h->Xb);
+  return half_sweep(h, 0);
 }
 
 int cumf_update_theta(cumf_handle* h) {
@@ -597,16 +697,19 @@ int cumf_update_theta(cumf_handle* h) {
     return CUMF_ERR_STATE;
   }
   DeviceGuard dg(h->device);
-  return run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb, h->Tb);
+  return half_sweep(h, 1);
 }
 
 namespace {
 
-// Captures one iteration (both half-sweeps' kernel launches, including the
-// warm start's features and network, no memsets or host work) on the
-// handle's stream into an executable graph.
-int capture_iteration(cumf_handle* h) {
-  h->drop_graph();
+// Captures one half-sweep (all its kernel launches, including the warm
+// start's features and network; no memsets or host work) on the handle's
+// stream into an executable graph.
+int capture_half_sweep(cumf_handle* h, int side) {
+  if (h->graph[side]) {
+    (void)hipGraphExecDestroy(h->graph[side]);
+    h->graph[side] = nullptr;
+  }
   CUMF_TRY(prepare_network(h));
   if (h->opt.solver == CUMF_SOLVER_CG) {
     const size_t need = sizeof(int) * (size_t)std::max<int64_t>(std::max(h->m, h->n), 1);
@@ -617,9 +720,10 @@ int capture_iteration(cumf_handle* h) {
   CUMF_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   h->capturing = true;
   const int64_t flops = h->gram_flops;
-  int rc = run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(), h->Tb, h->Xb);
-  if (rc == CUMF_OK)
-    rc = run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(), h->Xb, h->Tb);
+  const int rc = side == 0 ? run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(),
+                                            h->X.as<float>(), h->Tb, h->Xb)
+                           : run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(),
+                                            h->T.as<float>(), h->Xb, h->Tb);
   h->capturing = false;
   h->gram_flops = flops;  // counted again at each replay
   hipError_t e = hipStreamEndCapture(h->stream, &g);
@@ -631,14 +735,28 @@ int capture_iteration(cumf_handle* h) {
     cumf::set_last_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     return CUMF_ERR_HIP;
   }
-  e = hipGraphInstantiate(&h->iter_graph, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&h->graph[side], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    h->iter_graph = nullptr;
+    h->graph[side] = nullptr;
     cumf::set_last_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     return CUMF_ERR_HIP;
   }
   return CUMF_OK;
+}
+
+// One half-sweep: replayed from its graph with use_graph, else launched.
+int half_sweep(cumf_handle* h, int side) {
+  if (h->opt.use_graph) {
+    if (!h->graph[side]) CUMF_TRY(capture_half_sweep(h, side));
+    CUMF_TRY(timed(h, Timer::ITER, [&] { return hipGraphLaunch(h->graph[side], h->stream); }));
+    h->gram_flops += (side == 0 ? h->plan_x.nnz : h->plan_t.nnz) * (int64_t)h->f * (h->f + 1);
+    return CUMF_OK;
+  }
+  return side == 0 ? run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(),
+                                    h->Tb, h->Xb)
+                   : run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(),
+                                    h->Xb, h->Tb);
 }
 
 }  // namespace
@@ -649,18 +767,9 @@ int cumf_iterate(cumf_handle* h, int iters) {
     return CUMF_ERR_STATE;
   }
   if (iters < 0) return CUMF_ERR_INVALID;
-  if (!h->opt.use_graph) {
-    for (int it = 0; it < iters; ++it) {
-      CUMF_TRY(cumf_update_x(h));
-      CUMF_TRY(cumf_update_theta(h));
-    }
-    return CUMF_OK;
-  }
-  DeviceGuard dg(h->device);
-  if (!h->iter_graph) CUMF_TRY(capture_iteration(h));
   for (int it = 0; it < iters; ++it) {
-    CUMF_TRY(timed(h, Timer::ITER, [&] { return hipGraphLaunch(h->iter_graph, h->stream); }));
-    h->gram_flops += (h->plan_x.nnz + h->plan_t.nnz) * (int64_t)h->f * (h->f + 1);
+    CUMF_TRY(cumf_update_x(h));
+    CUMF_TRY(cumf_update_theta(h));
   }
   return CUMF_OK;
 }
@@ -779,6 +888,281 @@ int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t n
   }
   CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
   CUMF_HIP_TRY(hipMemcpy(out, dout.p, dout.bytes, hipMemcpyDeviceToHost));
+  return CUMF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int rebuild_plans(cumf_handle* h) {
+  h->drop_graph();
+  if (!h->have_ratings) return CUMF_OK;
+  CUMF_TRY(h->plan_x.build(h->rp_host.data(), h->m, h->tile_split[0] > 0 ? h->tile_split[0] : h->opt.split, h->f));
+  CUMF_TRY(h->plan_t.build(h->cp_host.data(), h->n, h->tile_split[1] > 0 ? h->tile_split[1] : h->opt.split, h->f));
+  return CUMF_OK;
+}
+
+// Minimum over reps of the stream time of launch() (milliseconds).
+template <class F>
+int time_min_ms(cumf_handle* h, int reps, F&& launch, double* out) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  CUMF_HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return CUMF_ERR_HIP;
+  }
+  double best = 1e300;
+  int rc = CUMF_OK;
+  for (int r = 0; r < reps && rc == CUMF_OK; ++r) {
+    (void)hipEventRecord(e0, h->stream);
+    rc = launch();
+    (void)hipEventRecord(e1, h->stream);
+    if (rc == CUMF_OK && hipEventSynchronize(e1) != hipSuccess) rc = CUMF_ERR_HIP;
+    float ms = 0.f;
+    if (rc == CUMF_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) best = std::min(best, (double)ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *out = best;
+  return rc;
+}
+
+struct SideBufs {
+  const Plan* p;
+  const DevBuf *rowp, *col, *val;
+  const float* theta;
+  float* x;
+  const DevBuf* theta_b;
+  DevBuf* x_b;
+};
+
+SideBufs side_bufs(cumf_handle* h, int side) {
+  if (side == 0) return {&h->plan_x, &h->rp, &h->ci, &h->rv, h->T.as<float>(), h->X.as<float>(), &h->Tb, &h->Xb};
+  return {&h->plan_t, &h->cp, &h->ri, &h->cv, h->X.as<float>(), h->T.as<float>(), &h->Xb, &h->Tb};
+}
+
+int autotune_side(cumf_handle* h, int side, int reps) {
+  const std::vector<int32_t>& rp = side == 0 ? h->rp_host : h->cp_host;
+  const int64_t rows = side == 0 ? h->m : h->n;
+  int64_t maxdeg = 0;
+  for (int64_t u = 0; u < rows; ++u) maxdeg = std::max<int64_t>(maxdeg, (int64_t)rp[u + 1] - rp[u]);
+  // 1. Split size: whole half-sweeps, one launch per kernel.
+  h->tiled = false;
+  static const int kSplits[] = {512, 1024, 2048, 4096, 8192};
+  double best = 1e300;
+  int best_split = h->opt.split;
+  for (int sp : kSplits) {
+    if (sp > 2 * maxdeg && sp != kSplits[0]) break;
+    h->tile_split[side] = sp;
+    CUMF_TRY(rebuild_plans(h));
+    double ms = 0;
+    CUMF_TRY(time_min_ms(h, reps, [&] {
+      SideBufs b = side_bufs(h, side);
+      return run_half_sweep(h, side, *b.p, *b.rowp, *b.col, *b.val, b.theta, b.x, *b.theta_b, *b.x_b);
+    }, &ms));
+    if (ms < best) {
+      best = ms;
+      best_split = sp;
+    }
+  }
+  h->tile_split[side] = best_split;
+  CUMF_TRY(rebuild_plans(h));
+  // 2. Launch shape per degree bucket.
+  SideBufs b = side_bufs(h, side);
+  cumf::SweepArgs a{};
+  CUMF_TRY(fill_args(h, *b.p, *b.rowp, *b.col, *b.val, b.theta, b.x, *b.theta_b, *b.x_b, nullptr, a));
+  a.warm = side == 0 ? std::min(h->opt.warm_start, (int)CUMF_WARM_ZERO) : CUMF_WARM_PREV;
+  const bool cg = h->opt.solver == CUMF_SOLVER_CG;
+  for (int k = 0; k < kBuckets; ++k) {
+    if (b.p->bucket_t1[k] <= b.p->bucket_t0[k]) continue;
+    double bk = 1e300;
+    BucketCfg bc;
+    for (int variant : {2, 1})
+      for (int pers : {0, h->num_cus * (variant == 1 ? 4 : 8)}) {
+        BucketCfg cfg{variant, pers};
+        double ms = 0;
+        CUMF_TRY(time_min_ms(h, reps, [&] {
+          if (cg) CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
+          if (pers) CUMF_HIP_TRY(cumf::launch_fallback_reset(h->counters.as<int>() + side * kBuckets + k, 1, h->stream));
+          CUMF_HIP_TRY(bucket_launch(h, side, *b.p, a, k, cfg));
+          return CUMF_OK;
+        }, &ms));
+        if (ms < bk) {
+          bk = ms;
+          bc = cfg;
+        }
+      }
+    h->tiles[side][k] = bc;
+  }
+  // 3. Register budget of the split-row finish and of the Cholesky fallback
+  //    (timed on a list of up to 8,192 light rows).
+  h->finish_variant[side] = 2;
+  if (b.p->nheavy > 0) {
+    double t2 = 0, t1 = 0;
+    auto fin = [&](int v) {
+      return [&, v] {
+        if (cg) CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
+        CUMF_HIP_TRY(finish_launch(a, v, h->stream));
+        return CUMF_OK;
+      };
+    };
+    CUMF_TRY(time_min_ms(h, reps, fin(2), &t2));
+    CUMF_TRY(time_min_ms(h, reps, fin(1), &t1));
+    h->finish_variant[side] = t1 < t2 ? 1 : 2;
+  }
+  h->fallback_variant[side] = 2;
+  if (cg) {
+    std::vector<int> list;
+    const int64_t step = std::max<int64_t>(1, rows / 8192);
+    for (int64_t u = 0; u < rows && (int64_t)list.size() < 8192; u += step)
+      if ((int64_t)rp[u + 1] - rp[u] <= b.p->split) list.push_back((int)u);
+    const int cnt = (int)list.size();
+    if (cnt > 0) {
+      double t2 = 0, t1 = 0;
+      auto fb = [&](int v) {
+        return [&, v] {
+          CUMF_HIP_TRY(hipMemcpyAsync(h->fb_list.p, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice,
+                                      h->stream));
+          CUMF_HIP_TRY(hipMemcpyAsync(h->fb_count.p, &cnt, sizeof(int), hipMemcpyHostToDevice, h->stream));
+          CUMF_HIP_TRY(fallback_launch(a, v, h->stream));
+          return CUMF_OK;
+        };
+      };
+      CUMF_TRY(time_min_ms(h, reps, fb(2), &t2));
+      CUMF_TRY(time_min_ms(h, reps, fb(1), &t1));
+      h->fallback_variant[side] = t1 < t2 ? 1 : 2;
+      CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
+    }
+  }
+  return CUMF_OK;
+}
+
+int copy_dev(DevBuf& dst, const DevBuf& src, hipStream_t s) {
+  if (!src.p) return CUMF_OK;
+  if (dst.bytes != src.bytes) CUMF_TRY(dst.alloc(src.bytes));
+  CUMF_HIP_TRY(hipMemcpyAsync(dst.p, src.p, src.bytes, hipMemcpyDeviceToDevice, s));
+  return CUMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cumf_save_tile_table(cumf_handle* h, const char* path) {
+  cumf::clear_last_error();
+  if (!h || !path) return CUMF_ERR_INVALID;
+  FILE* fp = std::fopen(path, "w");
+  if (!fp) {
+    cumf::set_last_error(std::string("cannot write ") + path);
+    return CUMF_ERR_INVALID;
+  }
+  std::fprintf(fp, "# cumf tile table: f %d precision %d solver %d\n", h->f, h->opt.precision, h->opt.solver);
+  std::fprintf(fp, "# side <0 X | 1 Theta> split <ratings per task> finish <1|2> fallback <1|2>\n");
+  std::fprintf(fp, "# bucket <side> <b> <min ratings> <max ratings> variant <1|2> persistent <wavefronts>\n");
+  for (int side = 0; side < 2; ++side) {
+    const Plan& p = side == 0 ? h->plan_x : h->plan_t;
+    std::fprintf(fp, "side %d split %d finish %d fallback %d\n", side, p.split > 0 ? p.split : h->opt.split,
+                 h->finish_variant[side], h->fallback_variant[side]);
+    for (int b = 0; b < kBuckets; ++b)
+      std::fprintf(fp, "bucket %d %d %lld %lld variant %d persistent %d\n", side, b, (long long)kBucketEdge[b],
+                   (long long)(b + 1 < kBuckets ? kBucketEdge[b + 1] - 1 : -1), h->tiles[side][b].variant,
+                   h->tiles[side][b].persistent);
+  }
+  const bool ok = std::fclose(fp) == 0;
+  if (!ok) cumf::set_last_error(std::string("cannot write ") + path);
+  return ok ? CUMF_OK : CUMF_ERR_INVALID;
+}
+
+int cumf_load_tile_table(cumf_handle* h, const char* path) {
+  cumf::clear_last_error();
+  if (!h || !path) return CUMF_ERR_INVALID;
+  FILE* fp = std::fopen(path, "r");
+  if (!fp) {
+    cumf::set_last_error(std::string("cannot read ") + path);
+    return CUMF_ERR_INVALID;
+  }
+  int split[2] = {0, 0}, fin[2] = {2, 2}, fb[2] = {2, 2};
+  BucketCfg tiles[2][kBuckets];
+  char line[512];
+  int rc = CUMF_OK, lineno = 0;
+  while (rc == CUMF_OK && std::fgets(line, sizeof line, fp)) {
+    ++lineno;
+    if (line[0] == '#' || line[0] == '\n') continue;
+    int side, sp, fv, bv, b, v, pers;
+    long long lo, hi;
+    if (std::sscanf(line, "side %d split %d finish %d fallback %d", &side, &sp, &fv, &bv) == 4) {
+      if (side < 0 || side > 1 || sp < 1 || (fv != 1 && fv != 2) || (bv != 1 && bv != 2)) rc = CUMF_ERR_INVALID;
+      else { split[side] = sp; fin[side] = fv; fb[side] = bv; }
+    } else if (std::sscanf(line, "bucket %d %d %lld %lld variant %d persistent %d", &side, &b, &lo, &hi, &v,
+                           &pers) == 6) {
+      if (side < 0 || side > 1 || b < 0 || b >= kBuckets || (v != 1 && v != 2) || pers < 0 || pers > (1 << 20))
+        rc = CUMF_ERR_INVALID;
+      else tiles[side][b] = {v, pers};
+    } else {
+      rc = CUMF_ERR_INVALID;
+    }
+  }
+  std::fclose(fp);
+  if (rc != CUMF_OK) {
+    cumf::set_last_error(std::string(path) + ": bad tile table line " + std::to_string(lineno));
+    return rc;
+  }
+  DeviceGuard dg(h->device);
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int side = 0; side < 2; ++side) {
+    h->tile_split[side] = split[side];
+    h->finish_variant[side] = fin[side];
+    h->fallback_variant[side] = fb[side];
+    for (int b = 0; b < kBuckets; ++b) h->tiles[side][b] = tiles[side][b];
+  }
+  h->tiled = true;
+  return rebuild_plans(h);
+}
+
+int cumf_autotune(cumf_handle* h, int reps, const char* path) {
+  cumf::clear_last_error();
+  if (!h) return CUMF_ERR_INVALID;
+  if (!h->have_ratings) {
+    cumf::set_last_error("cumf_autotune: no ratings yet");
+    return CUMF_ERR_STATE;
+  }
+  if (h->opt.compat_hermitian_bug) {
+    cumf::set_last_error("cumf_autotune: the compat path has a single launch shape");
+    return CUMF_ERR_INVALID;
+  }
+  reps = std::max(1, std::min(reps, 20));
+  DeviceGuard dg(h->device);
+  CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  // The timed sweeps solve real rows: keep the factors and put them back.
+  DevBuf sx, st, sxb, stb;
+  CUMF_TRY(copy_dev(sx, h->X, h->stream));
+  CUMF_TRY(copy_dev(st, h->T, h->stream));
+  CUMF_TRY(copy_dev(sxb, h->Xb, h->stream));
+  CUMF_TRY(copy_dev(stb, h->Tb, h->stream));
+  const bool timing = h->opt.timing;
+  const int warm = h->opt.warm_start;
+  h->opt.timing = 0;
+  if (warm >= CUMF_WARM_MLP) h->opt.warm_start = CUMF_WARM_PREV;  // the network is not part of the tile choice
+  int rc = autotune_side(h, 0, reps);
+  if (rc == CUMF_OK) rc = autotune_side(h, 1, reps);
+  h->opt.timing = timing;
+  h->opt.warm_start = warm;
+  if (rc == CUMF_OK) rc = copy_dev(h->X, sx, h->stream);
+  if (rc == CUMF_OK) rc = copy_dev(h->T, st, h->stream);
+  if (rc == CUMF_OK && sxb.p) rc = copy_dev(h->Xb, sxb, h->stream);
+  if (rc == CUMF_OK && stb.p) rc = copy_dev(h->Tb, stb, h->stream);
+  if (rc == CUMF_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = CUMF_ERR_HIP;
+  // Counters of the tuning runs are not statistics of the user's sweeps.
+  if (rc == CUMF_OK && hipMemset(h->fail.p, 0, 2 * sizeof(int)) != hipSuccess) rc = CUMF_ERR_HIP;
+  h->tiled = rc == CUMF_OK;
+  h->drop_graph();
+  if (rc != CUMF_OK) {
+    h->tile_split[0] = h->tile_split[1] = 0;
+    (void)rebuild_plans(h);
+    return rc;
+  }
+  if (path) CUMF_TRY(cumf_save_tile_table(h, path));
   return CUMF_OK;
 }
 
@@ -1081,8 +1465,8 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
     CUMF_HIP_TRY(cumf::prepare_compat(f));
     CUMF_HIP_TRY(cumf::launch_compat(c, m, stream));
   } else {
-    CUMF_HIP_TRY(cumf::launch_task_kernel(a, stream));
-    CUMF_HIP_TRY(cumf::launch_finish_kernel(a, stream));
+    CUMF_HIP_TRY(cumf::w2::launch_task_kernel(a, stream));
+    CUMF_HIP_TRY(cumf::w2::launch_finish_kernel(a, stream));
   }
   if (ldf != f)
     CUMF_HIP_TRY(hipMemcpy2DAsync(d_x, sizeof(float) * f, x, sizeof(float) * ldf, sizeof(float) * f, m,

@@ -1,0 +1,871 @@
+// als_device.h -- device-side building blocks of the ALS half-sweep kernels
+// (included by als_sweep.hip and als_kernels.hip).
+//
+// One half-sweep solves, for every row u of a CSR rating matrix,
+//     (Y_u^T Y_u + lambda n_u I) x_u = Y_u^T r_u
+// where Y_u stacks the n_u factor rows of the other side named by row u's
+// column indices (reference: get_hermitian100 / get_hermitianT10 at
+// als/src/cuda/als.cu:47-912, the cusparseScsrmm2 RHS at :1039-1054 and
+// the batched LU solve updateX at :935-997).
+//
+// One 64-lane wavefront per task (a row, or a chunk of a long row):
+//   * Gram: the factor rows of each chunk of ratings are gathered into LDS
+//     and the augmented Gram [Y r]^T [Y r] is accumulated on MFMA over the
+//     NB(NB+1)/2 upper 16x16 tiles (FP32: v_mfma_f32_16x16x4_f32; BF16:
+//     v_mfma_f32_16x16x32_bf16 with the rating split into a BF16 high and
+//     low column).  The extra column(s) carry b = Y^T r.
+//   * The accumulators stay in registers; their layout (lane l holds rows
+//     16I + 4(l/16) + {0..3} of column 16J + l%16) is a 2-D block-cyclic
+//     distribution on which a right-looking Cholesky runs in place, leaving
+//     z = U^{-T} b in the augmented column; U and z go to LDS for the back
+//     substitution.
+//   * Or the mixed-precision solve (cg_solve): FP16 CG on the Jacobi-scaled
+//     matrix in LDS inside FP32 refinement with an FP64 residual.
+//   * Rows with more than `split` ratings are cut into chunks whose partial
+//     Grams go to a scratch buffer in register order; a second kernel sums
+//     them in chunk order (deterministic) and runs the same solve.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "als_kernels.h"
+
+namespace cumf {
+
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) bf4 lds_bf4_t;
+
+constexpr int KB = 16;  // ratings per staged chunk
+
+
+__host__ __device__ constexpr int num_tiles(int NB) { return NB * (NB + 1) / 2; }
+__host__ __device__ constexpr int tile_index(int I, int J, int NB) {
+  return I * NB - I * (I - 1) / 2 + (J - I);
+}
+
+// LDS floats per wave: the staging buffers of the Gram phase and the
+// row/column buffers of the solve share the same space.
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int lds_floats_nb(int NB, int f) {
+  return cmax(cmax(2 * KB * ((f + 3) & ~3) + 2 * KB,   // FP32 staging
+                   2 * 32 * 16 * NB / 2),              // BF16 staging (2 x 32 rows x 16NB bf16)
+              cmax(f * (f + 1) / 2 + 2 * f, 32 * NB));  // U, z, z2; two pivot-row buffers
+}
+
+
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// Waits for every outstanding vector-memory operation of this wave,
+// including LDS-DMA loads (global_load_lds), whose completion the compiler
+// does not track; also a compiler memory barrier.
+__device__ __forceinline__ void wait_vm_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Orders LDS accesses of one wave (LDS executes a wave's DS instructions in
+// issue order, so only the compiler must be kept from reordering).
+__device__ __forceinline__ void lds_order() {
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// ---------------------------------------------------------------------------
+// Gram accumulation over a contiguous range of one row's ratings.
+//
+// Staging: two LDS buffers of KB factor rows (row-major, stride ldf) plus
+// KB ratings.  Factor rows are gathered straight into LDS with
+// global_load_lds_dwordx4 (lane l of load `it` moves 16 bytes of slot
+// it*64+l = (row, quad) to LDS offset 16*(it*64+l), i.e. row*ldf+4*quad),
+// one chunk ahead of the MFMAs.  Rows past the end of the range are
+// zeroed when the MFMA operands are read.
+// ---------------------------------------------------------------------------
+template <int NB>
+__device__ __forceinline__ void gram_range(f4 (&acc)[num_tiles(NB)], const int32_t* __restrict__ col,
+                                           const float* __restrict__ val,
+                                           const float* __restrict__ theta, int64_t ldf,
+                                           int64_t start, int count, float* __restrict__ lds,
+                                           int lane) {
+  const int g = lane >> 4, c = lane & 15;
+  const int ld = (int)ldf;
+  const int Q = ld >> 2;       // 16-byte quads per factor row
+  const int slots = KB * Q;    // quads per chunk
+  const int cf = KB * ld;      // floats per staged chunk
+  float* rb = lds + 2 * cf;    // ratings, [2][KB]
+  // Last column block: factor columns, then the rating column (ldf), then zeros.
+  const int lastc = 16 * (NB - 1) + c;
+  const int lastkind = lastc < ld ? 0 : (lastc == ld ? 1 : 2);
+  const int lastoff = lastc < ld ? lastc : ld - 1;
+  const int nch = (count + KB - 1) / KB;
+
+  auto load_ci = [&](int ch) -> int {
+    const int k = ch * KB + lane;
+    return (lane < KB && ch < nch && k < count) ? col[start + k] : 0;
+  };
+  auto issue = [&](int ch, int ci) {
+    float* yb = lds + (ch & 1) * cf;
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+      if (it * 64 < slots) {
+        const int sl = it * 64 + lane;
+        const int row = sl / Q;
+        const int q = sl - row * Q;
+        const int src = __shfl(ci, row < KB ? row : 0);
+        if (sl < slots)
+          __builtin_amdgcn_global_load_lds(theta + (int64_t)src * ldf + 4 * q,
+                                           (lds_void_t*)(yb + it * 256), 16, 0, 0);
+      }
+    }
+    const int k = ch * KB + lane;
+    if (lane < KB && k < count)
+      __builtin_amdgcn_global_load_lds(val + start + k, (lds_void_t*)(rb + (ch & 1) * KB), 4, 0, 0);
+  };
+
+  int ci_next = load_ci(0);
+  issue(0, ci_next);
+  ci_next = load_ci(1);
+  for (int ch = 0; ch < nch; ++ch) {
+    wait_vm_all();  // chunk ch is in LDS; column indices of ch+1 are in ci_next
+    if (ch + 1 < nch) issue(ch + 1, ci_next);
+    ci_next = load_ci(ch + 2);
+    const float* yb = lds + (ch & 1) * cf;
+    const float* rr = rb + (ch & 1) * KB;
+    const int valid = count - ch * KB;
+#pragma unroll
+    for (int s = 0; s < KB / 4; ++s) {
+      const int row = 4 * s + g;
+      const bool ok = row < valid;
+      float a[NB];
+#pragma unroll
+      for (int I = 0; I < NB - 1; ++I) a[I] = ok ? yb[row * ld + 16 * I + c] : 0.0f;
+      const float yl = yb[row * ld + lastoff];
+      const float rl = rr[row];
+      a[NB - 1] = !ok ? 0.0f : (lastkind == 0 ? yl : (lastkind == 1 ? rl : 0.0f));
+#pragma unroll
+      for (int I = 0; I < NB; ++I)
+#pragma unroll
+        for (int J = I; J < NB; ++J)
+          acc[tile_index(I, J, NB)] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+              a[I], a[J], acc[tile_index(I, J, NB)], 0, 0, 0);
+    }
+    lds_order();
+  }
+  wait_vm_all();
+}
+
+// ---------------------------------------------------------------------------
+// BF16 Gram (precision mode): the same augmented product on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+//
+// Factor rows come from a BF16 shadow copy (row stride ldb = round_up(f,8),
+// zero padded).  A chunk of 32 ratings is staged in LDS as 32 rows of
+// W = 16*NB bf16: 2*NB 16-byte slots per row, moved with
+// global_load_lds_dwordx4 (slots past the factor row, and every slot of a
+// row past the end of the range, come from a zero block).  The rating r is
+// written into columns f and f+1 as bf16(r) and bf16(r - bf16(r)), so
+// b = Y^T r keeps ~16 significant bits (exact for integer ratings).
+//
+// MFMA operands come straight from the row-major tile with
+// ds_read_b64_tr_b16: within a 16-lane group, lane j addresses 4 bf16 of
+// row (8*kg + j/4), columns 16I + 4*(j%4); the transpose read returns to
+// lane i the element (i%4) of the segment of lane 4s + i/4, i.e. column
+// 16I + i of rows 8kg + s, s = 0..3.  A second read 4 rows further gives
+// the 8 consecutive k values the 16x16x32 operand needs.
+// ---------------------------------------------------------------------------
+constexpr int KBB = 32;  // ratings per BF16 chunk (one MFMA k-step)
+
+template <int NB>
+__device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const int32_t* __restrict__ col,
+                                                const float* __restrict__ val,
+                                                const __bf16* __restrict__ theta_b, int64_t ldb,
+                                                const void* __restrict__ zero16, int f, int64_t start,
+                                                int count, __bf16* __restrict__ lds, int lane) {
+  constexpr int W = 16 * NB;        // bf16 per staged row
+  constexpr int SPR = 2 * NB;       // 16-byte slots per staged row
+  constexpr int CB = KBB * W;       // bf16 per staged chunk
+  const int real = (int)(ldb >> 3);  // slots carrying factor data
+  const int kg = lane >> 4, j = lane & 15;
+  const int nch = (count + KBB - 1) / KBB;
+  // this lane's transpose-read offsets inside a chunk (bf16 units)
+  const int troff = (8 * kg + (j >> 2)) * W + 4 * (j & 3);
+
+  auto load_ci = [&](int ch) -> int {
+    const int k = ch * KBB + lane;
+    return (lane < KBB && ch < nch && k < count) ? col[start + k] : -1;
+  };
+  auto load_r = [&](int ch) -> float {
+    const int k = ch * KBB + lane;
+    return (lane < KBB && ch < nch && k < count) ? val[start + k] : 0.0f;
+  };
+  auto issue = [&](int ch, int ci) {
+    __bf16* yb = lds + (ch & 1) * CB;
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+      const int sl = it * 64 + lane;  // NB*64 = KBB*SPR slots exactly
+      const int row = sl / SPR;
+      const int q = sl - row * SPR;
+      const int src = __shfl(ci, row);
+      const void* g = (src >= 0 && q < real) ? (const void*)(theta_b + (int64_t)src * ldb + 8 * q) : zero16;
+      __builtin_amdgcn_global_load_lds(g, (lds_void_t*)(yb + it * 512), 16, 0, 0);
+    }
+  };
+
+  int ci = load_ci(0);
+  float rv = load_r(0);
+  issue(0, ci);
+  ci = load_ci(1);
+  float rnext = load_r(1);
+  for (int ch = 0; ch < nch; ++ch) {
+    wait_vm_all();  // chunk ch staged; indices and ratings of ch+1 loaded
+    __bf16* yb = lds + (ch & 1) * CB;
+    if (lane < KBB && ch * KBB + lane < count) {
+      const __bf16 hi = (__bf16)rv;
+      const __bf16 lo = (__bf16)(rv - (float)hi);
+      yb[lane * W + f] = hi;
+      yb[lane * W + f + 1] = lo;
+    }
+    if (ch + 1 < nch) issue(ch + 1, ci);
+    rv = rnext;
+    ci = load_ci(ch + 2);
+    rnext = load_r(ch + 2);
+    lds_order();
+    bf8 op[NB];
+#pragma unroll
+    for (int I = 0; I < NB; ++I) {
+      const bf4 lo4 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf4_t*)(yb + troff + 16 * I));
+      const bf4 hi4 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf4_t*)(yb + troff + 4 * W + 16 * I));
+      op[I] = __builtin_shufflevector(lo4, hi4, 0, 1, 2, 3, 4, 5, 6, 7);
+    }
+#pragma unroll
+    for (int I = 0; I < NB; ++I)
+#pragma unroll
+      for (int J = I; J < NB; ++J)
+        acc[tile_index(I, J, NB)] =
+            __builtin_amdgcn_mfma_f32_16x16x32_bf16(op[I], op[J], acc[tile_index(I, J, NB)], 0, 0, 0);
+    lds_order();
+  }
+  wait_vm_all();
+}
+
+// Right-looking Cholesky on the accumulator layout, software-pipelined by
+// one pivot.  Step k uses the pivot row a_k* (its diagonal d and the values
+// colv[J] = a_k,16J+c were read during step k-1), then:
+//   1. computes row k+1's updated values first (temporaries) and has its
+//      16 owner lanes broadcast them through the other half of a two-row
+//      LDS buffer, and issues the reads of the next pivot and of colv;
+//   2. applies the rank-1 update to every element it owns (the same fused
+//      multiply-adds, so row k+1 ends bit-identical to its broadcast copy).
+// The LDS round trip of pivot k+1 thus overlaps the bulk update of pivot k.
+// K (block row) and KR (register of the pivot row) are compile-time, so
+// every accumulator access is a fixed register; the 16-lane group kg that
+// owns row k is a run-time loop.
+
+// LASTG: kg == 3 is known at compile time, so the pivot after KR == 3 is
+// the first row of block K+1 (a fixed register) rather than of group kg+1.
+// The pivot row's values a_k,16I+4g+{0..3} are streamed block by block
+// (one block in use, one prefetched) to keep the register footprint small.
+template <int NB, int K, int KR, bool LASTG>
+__device__ __forceinline__ void chol_step(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ rb, bool& ok,
+                                          int lane, int kg, float& d, float (&colv)[NB]) {
+  const int g = lane >> 4, c = lane & 15;
+  const int k = 16 * K + 4 * kg + KR;
+  if (k >= f) return;  // uniform
+  const float* cur = rb + (k & 1) * 16 * NB;
+  float* nxt = rb + ((k + 1) & 1) * 16 * NB;
+  f4 rvK = *reinterpret_cast<const f4*>(cur + 16 * K + 4 * g);
+  f4 rvN = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (K + 1 < NB) rvN = *reinterpret_cast<const f4*>(cur + 16 * (K + 1) + 4 * g);
+  const bool good = d > 0.0f && d < __builtin_huge_valf();
+  ok = ok && good;
+  const float inv = good ? __builtin_amdgcn_rsqf(d) : 0.0f;  // 1/sqrt(d)
+  const float inv2 = inv * inv;                               // 1/d
+  float cj[NB];
+#pragma unroll
+  for (int J = K; J < NB; ++J) cj[J] = colv[J] * inv2;
+  // Rows of block K at or above k are finished; later blocks are all below k.
+#pragma unroll
+  for (int r = 0; r < 4; ++r) rvK[r] = (4 * g + r > 4 * kg + KR) ? rvK[r] : 0.0f;
+  if (g == kg) {  // row k of U
+#pragma unroll
+    for (int J = K; J < NB; ++J) acc[tile_index(K, J, NB)][KR] = colv[J] * inv;
+  }
+  if (k + 1 < f) {
+    // Row k+1's owners: same block and group (KR < 3), the next group, or
+    // group 0 of the next block.
+    constexpr int R1 = KR < 3 ? KR + 1 : 0;
+    if constexpr (KR < 3 || !LASTG) {
+      const int g1 = KR < 3 ? kg : kg + 1;
+      if (g == g1) {
+#pragma unroll
+        for (int J = K; J < NB; ++J) nxt[16 * J + c] = fmaf(-rvK[R1], cj[J], acc[tile_index(K, J, NB)][R1]);
+      }
+    } else if constexpr (K + 1 < NB) {
+      if (g == 0) {
+#pragma unroll
+        for (int J = K + 1; J < NB; ++J)
+          nxt[16 * J + c] = fmaf(-rvN[0], cj[J], acc[tile_index(K + 1, J, NB)][0]);
+      }
+    }
+    lds_order();
+    d = nxt[k + 1];
+#pragma unroll
+    for (int J = K; J < NB; ++J) colv[J] = nxt[16 * J + c];
+  }
+  // Rank-1 update of every element owned, block row by block row.
+  f4 rv = rvK;  // block I
+  f4 nx = rvN;  // block I+1
+#pragma unroll
+  for (int I = K; I < NB; ++I) {
+    f4 nn = {0.f, 0.f, 0.f, 0.f};  // block I+2
+    if (I + 2 < NB) nn = *reinterpret_cast<const f4*>(cur + 16 * (I + 2) + 4 * g);
+#pragma unroll
+    for (int J = I; J < NB; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        acc[tile_index(I, J, NB)][r] = fmaf(-rv[r], cj[J], acc[tile_index(I, J, NB)][r]);
+    rv = nx;
+    nx = nn;
+  }
+}
+
+template <int NB, int K>
+__device__ __forceinline__ void chol_block(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ rb, bool& ok,
+                                           int lane, float& d, float (&colv)[NB]) {
+  if (16 * K < f) {
+#pragma unroll 1
+    for (int kg = 0; kg < 3; ++kg) {
+      chol_step<NB, K, 0, false>(acc, f, rb, ok, lane, kg, d, colv);
+      chol_step<NB, K, 1, false>(acc, f, rb, ok, lane, kg, d, colv);
+      chol_step<NB, K, 2, false>(acc, f, rb, ok, lane, kg, d, colv);
+      chol_step<NB, K, 3, false>(acc, f, rb, ok, lane, kg, d, colv);
+    }
+    chol_step<NB, K, 0, true>(acc, f, rb, ok, lane, 3, d, colv);
+    chol_step<NB, K, 1, true>(acc, f, rb, ok, lane, 3, d, colv);
+    chol_step<NB, K, 2, true>(acc, f, rb, ok, lane, 3, d, colv);
+    chol_step<NB, K, 3, true>(acc, f, rb, ok, lane, 3, d, colv);
+  }
+  if constexpr (K + 1 < NB) chol_block<NB, K + 1>(acc, f, rb, ok, lane, d, colv);
+}
+
+// bcol2 >= 0: a second right-hand-side column whose solution is added to
+// the first (the BF16 low part of the ratings).  xb: optional BF16 copy of
+// the solution (row of ldb entries, zero padded).
+template <int NB>
+__device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int bcol,
+                                           float* __restrict__ lds, float* __restrict__ xrow,
+                                           int64_t ldf, int lane, int bcol2 = -1,
+                                           __bf16* __restrict__ xb = nullptr, int64_t ldb = 0) {
+  const int g = lane >> 4, c = lane & 15;
+  bool ok = true;
+  {
+    // Pivot row 0 (group 0, register 0 of tiles (0, J)) into the first buffer.
+    if (g == 0) {
+#pragma unroll
+      for (int J = 0; J < NB; ++J) lds[16 * J + c] = acc[tile_index(0, J, NB)][0];
+    }
+    lds_order();
+    float d = lds[0];
+    float colv[NB];
+#pragma unroll
+    for (int J = 0; J < NB; ++J) colv[J] = lds[16 * J + c];
+    chol_block<NB, 0>(acc, f, lds, ok, lane, d, colv);
+  }
+
+  // Spill U (packed by columns: U[i][j] at j(j+1)/2 + i) and z.
+  const int upk = f * (f + 1) / 2;
+  float* ucol = lds;
+  float* zbuf = lds + upk;
+  float* zbuf2 = zbuf + f;
+  lds_order();
+#pragma unroll
+  for (int I = 0; I < NB; ++I)
+#pragma unroll
+    for (int J = I; J < NB; ++J) {
+      const int j = 16 * J + c;
+      const int cbase = j * (j + 1) / 2;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * I + 4 * g + r;
+        const float v = acc[tile_index(I, J, NB)][r];
+        if (j < f && i <= j) ucol[cbase + i] = v;
+        if (j == bcol && i < f) zbuf[i] = v;
+        if (j == bcol2 && i < f) zbuf2[i] = v;
+      }
+    }
+  lds_order();
+
+  // Back substitution, one row index per lane (f <= 192).
+  float z[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int i = lane + 64 * q;
+    z[q] = (i < f) ? zbuf[i] + (bcol2 >= 0 ? zbuf2[i] : 0.0f) : 0.0f;
+  }
+  for (int k = f - 1; k >= 0; --k) {
+    const int kq = k >> 6, kl = k & 63;
+    const float zsel = kq == 0 ? z[0] : (kq == 1 ? z[1] : z[2]);
+    const float zk = readlane_f(zsel, kl);
+    const float xk = zk / ucol[k * (k + 1) / 2 + k];
+    const float* uk = ucol + k * (k + 1) / 2;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int i = lane + 64 * q;
+      if (i < k) z[q] = fmaf(-uk[i], xk, z[q]);
+      if (i == k) z[q] = xk;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int i = lane + 64 * q;
+    const float xv = (ok && i < f) ? z[q] : 0.0f;
+    if (i < ldf) xrow[i] = xv;
+    if (xb && i < ldb) xb[i] = (__bf16)xv;
+  }
+  return ok;
+}
+
+template <int NB>
+__device__ __forceinline__ void add_ridge(f4 (&acc)[num_tiles(NB)], int f, float ridge, int lane) {
+  const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+  for (int I = 0; I < NB; ++I)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (c == 4 * g + r && 16 * I + c < f) acc[tile_index(I, I, NB)][r] += ridge;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision solve: conjugate gradients on the Jacobi-scaled matrix
+// A^ = D^-1/2 A D^-1/2 held in LDS as FP16 (unit diagonal, |entries| <= 1,
+// so FP16 holds it without range trouble), inside FP32 iterative
+// refinement whose residual r = b - A x is formed in FP64 from the FP32
+// Gram still in registers.  Each refinement round solves A^ d = D^-1/2 r /
+// mu (mu = max |D^-1/2 r|, so the FP16 search directions stay O(1)) with a
+// fixed number of CG iterations from d = 0 and adds mu D^-1/2 d to x.  The
+// start is the row's current factor (the previous sweep's value).  A row
+// whose relative residual ||b - A x|| / ||b|| has not reached the tolerance
+// after the last round is solved again by the Cholesky path.
+//
+// LDS (after the Gram phase): A^ (f rows, stride cg_stride(f) halves), the
+// FP16 direction p, x (16NB floats) and y = A x (16NB doubles).  Vectors
+// live one entry per lane (entry i in lane i % 64, slot i / 64).
+// ---------------------------------------------------------------------------
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+// Row stride of A^ in halves: f rounded up to 8, then to an odd multiple of
+// 8, so that 16-byte row reads of consecutive lanes hit distinct banks.
+__host__ __device__ constexpr int cg_stride(int f) {
+  return (((f + 7) / 8) & 1) ? ((f + 7) & ~7) : ((f + 7) & ~7) + 8;
+}
+__host__ __device__ constexpr int cg_lds_floats(int NB, int f) {
+  return (f + 1) * cg_stride(f) / 2 + 16 * NB + 32 * NB;
+}
+__host__ __device__ constexpr int rows_per_lane(int NB) { return (16 * NB + 63) / 64; }
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// Sum / max over the wave, returned uniform: quad butterflies and two row
+// rotations give every lane its 16-lane row total; the four row totals are
+// combined in a fixed order from lanes 0, 16, 32, 48.
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x124>(v);  // row_ror:4
+  v += dpp_f<0x128>(v);  // row_ror:8
+  return (readlane_f(v, 0) + readlane_f(v, 16)) + (readlane_f(v, 32) + readlane_f(v, 48));
+}
+__device__ __forceinline__ float wave_max(float v) {
+  v = fmaxf(v, dpp_f<0xB1>(v));
+  v = fmaxf(v, dpp_f<0x4E>(v));
+  v = fmaxf(v, dpp_f<0x124>(v));
+  v = fmaxf(v, dpp_f<0x128>(v));
+  return fmaxf(fmaxf(readlane_f(v, 0), readlane_f(v, 16)), fmaxf(readlane_f(v, 32), readlane_f(v, 48)));
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return readlane_d(v, 0);
+}
+
+// A register value the compiler must treat as changed at this point: keeps
+// loop-invariant FP64 conversions of the Gram from being hoisted out of the
+// refinement loop (which would hold 2x the Gram's registers live).
+__device__ __forceinline__ f4 opaque(f4 v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// ys[0, 16NB) = A xs in FP64, A the Gram in registers (upper tiles; the
+// diagonal tiles are full), xs zero beyond f.  Row direction: lane (g, c)
+// of tile (I, J) adds A[16I+4g+r][16J+c] x[16J+c] to row 16I+4g+r; the 16
+// lanes of a group are reduced with a reduce-scatter that leaves row
+// 16I+4g+(c>>2) in lanes c with equal c>>2.  Column direction (strictly
+// upper tiles, the lower triangle by symmetry): row 16J+c collects
+// A[16I+4g+r][16J+c] x[16I+4g+r] over r, I < J and the four groups.
+template <int NB>
+__device__ __forceinline__ void gram_matvec_f64(const f4 (&acc)[num_tiles(NB)], const float* __restrict__ xs,
+                                                double* __restrict__ ys, int lane) {
+  const int g = lane >> 4, c = lane & 15;
+  double xc[NB];
+#pragma unroll
+  for (int J = 0; J < NB; ++J) xc[J] = (double)xs[16 * J + c];
+  const bool h8 = (c & 8) != 0, h4b = (c & 4) != 0;
+#pragma unroll
+  for (int I = 0; I < NB; ++I) {
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+#pragma unroll
+    for (int J = I; J < NB; ++J) {
+      const f4 v = opaque(acc[tile_index(I, J, NB)]);
+      y0 = fma((double)v[0], xc[J], y0);
+      y1 = fma((double)v[1], xc[J], y1);
+      y2 = fma((double)v[2], xc[J], y2);
+      y3 = fma((double)v[3], xc[J], y3);
+    }
+    double k0 = h8 ? y2 : y0, k1 = h8 ? y3 : y1;
+    const double s0 = h8 ? y0 : y2, s1 = h8 ? y1 : y3;
+    k0 += __shfl_xor(s0, 8, 64);
+    k1 += __shfl_xor(s1, 8, 64);
+    double k = h4b ? k1 : k0;
+    const double s = h4b ? k0 : k1;
+    k += __shfl_xor(s, 4, 64);
+    k += __shfl_xor(k, 2, 64);
+    k += __shfl_xor(k, 1, 64);
+    if ((c & 3) == 0) ys[16 * I + 4 * g + (c >> 2)] = k;
+  }
+  lds_order();
+#pragma unroll
+  for (int J = 1; J < NB; ++J) {
+    double yc = 0.0;
+#pragma unroll
+    for (int I = 0; I < J; ++I) {
+      const f4 xr = *reinterpret_cast<const f4*>(xs + 16 * I + 4 * g);
+      const f4 v = opaque(acc[tile_index(I, J, NB)]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) yc = fma((double)v[r], (double)xr[r], yc);
+    }
+    yc += __shfl_xor(yc, 16, 64);
+    yc += __shfl_xor(yc, 32, 64);
+    if (g == 0) ys[16 * J + c] += yc;
+  }
+  lds_order();
+}
+
+// q = A^ p for the rows this lane owns (FP16 operands, FP32 accumulation).
+// Rows past f are computed on row f-1 (same-address reads broadcast) and
+// zeroed.
+template <int NB>
+__device__ __forceinline__ void cg_matvec(const _Float16* __restrict__ Ah, const _Float16* __restrict__ p16,
+                                          int f, int sa, int nch, int lane, float (&q)[rows_per_lane(NB)]) {
+  constexpr int RPL = rows_per_lane(NB);
+  const _Float16* rowp[RPL];
+  float q2[RPL];
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int i = lane + 64 * t;
+    rowp[t] = Ah + (i < f ? i : f - 1) * sa;
+    q[t] = 0.0f;
+    q2[t] = 0.0f;
+  }
+#pragma unroll 2
+  for (int k = 0; k < nch; ++k) {
+    const h8 pk = *reinterpret_cast<const h8*>(p16 + 8 * k);
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const h8 av = *reinterpret_cast<const h8*>(rowp[t] + 8 * k);
+      q[t] = __builtin_amdgcn_fdot2(h2{av[0], av[1]}, h2{pk[0], pk[1]}, q[t], false);
+      q2[t] = __builtin_amdgcn_fdot2(h2{av[2], av[3]}, h2{pk[2], pk[3]}, q2[t], false);
+      q[t] = __builtin_amdgcn_fdot2(h2{av[4], av[5]}, h2{pk[4], pk[5]}, q[t], false);
+      q2[t] = __builtin_amdgcn_fdot2(h2{av[6], av[7]}, h2{pk[6], pk[7]}, q2[t], false);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) q[t] = (lane + 64 * t < f) ? q[t] + q2[t] : 0.0f;
+}
+
+struct CgParams {
+  int iters;   // CG iterations per refinement round
+  int rounds;  // refinement rounds
+  float tol;   // accept x once ||b - A x|| <= tol ||b||
+  int warm;    // start: 0 the row's current factor, 1 zero, 2 x0row, 3 the better of 0 and 2
+};
+
+// Squared FP64 residual norm of the x in xs (lane-per-row b in bv).
+template <int NB>
+__device__ __forceinline__ double residual_norm2(const f4 (&acc)[num_tiles(NB)], const float* xs, double* ys,
+                                                 const float (&bv)[rows_per_lane(NB)], int f, int lane) {
+  gram_matvec_f64<NB>(acc, xs, ys, lane);
+  double rr = 0.0;
+#pragma unroll
+  for (int t = 0; t < rows_per_lane(NB); ++t) {
+    const int i = lane + 64 * t;
+    const double r = i < f ? (double)bv[t] - ys[i] : 0.0;
+    rr += r * r;
+  }
+  return wave_sum_d(rr);
+}
+
+// Returns true (and writes x to xrow / xb) if x was accepted; false leaves
+// the row to the Cholesky fallback kernel.
+// x0row: the network's prediction for this row (warm 2 and 3).  trace
+// (optional): initial and final relative residual, rounds run, fell back.
+template <int NB>
+__device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bcol, float* __restrict__ lds,
+                                         float* __restrict__ xrow, int64_t ldf, int lane, int bcol2,
+                                         __bf16* __restrict__ xb, int64_t ldb, const CgParams& cp,
+                                         const float* __restrict__ x0row, float* __restrict__ trace) {
+  constexpr int RPL = rows_per_lane(NB);
+  constexpr int D = 16 * NB;
+  const int g = lane >> 4, c = lane & 15;
+  const int sa = cg_stride(f);
+  const int f8 = (f + 7) & ~7;
+  _Float16* Ah = reinterpret_cast<_Float16*>(lds);
+  _Float16* p16 = Ah + f * sa;
+  float* xs = lds + (f + 1) * sa / 2;
+  double* ys = reinterpret_cast<double*>(xs + D);
+  float* ds = xs;                                  // diagonal, during set-up
+  float* bs = reinterpret_cast<float*>(ys);        // b (and the BF16 low part at bs + D)
+  float* ss = reinterpret_cast<float*>(ys);        // Jacobi scales, after b is read
+
+  // Diagonal and right-hand side out of the register tiles.
+  lds_order();
+#pragma unroll
+  for (int I = 0; I < NB; ++I)
+#pragma unroll
+    for (int J = I; J < NB; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * I + 4 * g + r, j = 16 * J + c;
+        const float v = acc[tile_index(I, J, NB)][r];
+        if (I == J && i == j && i < f) ds[i] = v;
+        if (j == bcol && i < f) bs[i] = v;
+        if (j == bcol2 && i < f) bs[D + i] = v;
+      }
+  lds_order();
+  float sc[RPL], bv[RPL], xv[RPL], xm[RPL];
+  bool bad = false, mbad = false;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int i = lane + 64 * t;
+    const bool in = i < f;
+    const float d = in ? ds[i] : 1.0f;
+    const bool good = d > 0.0f && d < __builtin_huge_valf();
+    bad = bad || !good;
+    sc[t] = (in && good) ? __builtin_amdgcn_rsqf(d) : 0.0f;
+    bv[t] = in ? bs[i] + (bcol2 >= 0 ? bs[D + i] : 0.0f) : 0.0f;
+    float x0 = (in && cp.warm != 1) ? xrow[i] : 0.0f;
+    if (!(fabsf(x0) < __builtin_huge_valf())) x0 = 0.0f;
+    xv[t] = x0;
+    xm[t] = (in && cp.warm >= 2) ? x0row[i] : 0.0f;
+    mbad = mbad || !(fabsf(xm[t]) < __builtin_huge_valf());
+  }
+  const bool net_bad = cp.warm >= 2 && __any(mbad);
+  // A non-finite prediction is never used: with warm 2 the row goes to the
+  // Cholesky solve, with warm 3 the current factor is the start.
+  const bool any_bad = __any(bad) || (cp.warm == 2 && net_bad);
+  if (cp.warm == 2 && !net_bad) {
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) xv[t] = xm[t];
+  }
+  lds_order();
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int i = lane + 64 * t;
+    if (i < D) {
+      ss[i] = sc[t];
+      xs[i] = xv[t];
+    }
+  }
+  lds_order();
+  bool accepted = false;
+  float rel0 = __builtin_nanf(""), rel = __builtin_nanf("");
+  int rounds_run = 0;
+  if (!any_bad) {
+    // A^ in FP16: tile (I, J) writes rows 16I+4g+r at column 16J+c and, for
+    // I < J, row 16J+c at columns 16I+4g..+3 (one 8-byte store).
+#pragma unroll
+    for (int I = 0; I < NB; ++I) {
+      const f4 sr = *reinterpret_cast<const f4*>(ss + 16 * I + 4 * g);
+#pragma unroll
+      for (int J = I; J < NB; ++J) {
+        const int j = 16 * J + c;
+        const float scj = ss[j];
+        const f4 v = acc[tile_index(I, J, NB)];
+        h4 tv;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tv[r] = (_Float16)(v[r] * sr[r] * scj);
+        if (j < f8) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (16 * I + 4 * g + r < f) Ah[(16 * I + 4 * g + r) * sa + j] = tv[r];
+        }
+        if (I < J && j < f && 16 * I + 4 * g < f8) *reinterpret_cast<h4*>(Ah + j * sa + 16 * I + 4 * g) = tv;
+      }
+    }
+    double bb = 0.0;
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) bb += (double)bv[t] * (double)bv[t];
+    bb = wave_sum_d(bb);
+    const double tol2 = (double)cp.tol * (double)cp.tol * bb;
+    const int nch = f8 / 8;
+    if (cp.warm == 3 && !net_bad) {
+      // Start from the better of the current factor (in xs) and the prediction.
+      const double rp = residual_norm2<NB>(acc, xs, ys, bv, f, lane);
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int i = lane + 64 * t;
+        if (i < D) xs[i] = i < f ? xm[t] : 0.0f;
+      }
+      lds_order();
+      const double rm = residual_norm2<NB>(acc, xs, ys, bv, f, lane);
+      const bool use_net = rm < rp;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int i = lane + 64 * t;
+        if (use_net) xv[t] = xm[t];
+        if (i < D) xs[i] = i < f ? xv[t] : 0.0f;
+      }
+      lds_order();
+    }
+    for (int round = 0;; ++round) {
+      // FP64 residual of the current x (xs holds it).
+      gram_matvec_f64<NB>(acc, xs, ys, lane);
+      float rh[RPL];
+      double rr = 0.0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int i = lane + 64 * t;
+        const double r = i < f ? (double)bv[t] - ys[i] : 0.0;
+        rr += r * r;
+        rh[t] = (float)r * sc[t];
+      }
+      rr = wave_sum_d(rr);
+      if (round == 0) rel0 = (float)sqrt(rr / bb);
+      rel = (float)sqrt(rr / bb);
+      rounds_run = round;
+      if (rr <= tol2) {
+        accepted = true;
+        break;
+      }
+      if (round == cp.rounds) break;
+      float mu = 0.0f;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) mu = fmaxf(mu, fabsf(rh[t]));
+      mu = wave_max(mu);
+      if (!(mu > 0.0f && mu < __builtin_huge_valf())) break;
+      const float imu = 1.0f / mu;
+      float dv[RPL], rv[RPL], pv[RPL], q[RPL];
+      float rho = 0.0f;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        rv[t] = rh[t] * imu;
+        pv[t] = rv[t];
+        dv[t] = 0.0f;
+        rho = fmaf(rv[t], rv[t], rho);
+      }
+      rho = wave_sum(rho);
+      for (int it = 0; it < cp.iters; ++it) {
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int i = lane + 64 * t;
+          if (i < f8) p16[i] = (_Float16)pv[t];
+        }
+        lds_order();
+        cg_matvec<NB>(Ah, p16, f, sa, nch, lane, q);
+        float pq = 0.0f;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) pq = fmaf(pv[t], q[t], pq);
+        pq = wave_sum(pq);
+        if (!(pq > 0.0f) || !(rho > 0.0f)) break;
+        const float alpha = rho / pq;
+        float rn = 0.0f;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          dv[t] = fmaf(alpha, pv[t], dv[t]);
+          rv[t] = fmaf(-alpha, q[t], rv[t]);
+          rn = fmaf(rv[t], rv[t], rn);
+        }
+        rn = wave_sum(rn);
+        const float beta = rn / rho;
+        rho = rn;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) pv[t] = fmaf(beta, pv[t], rv[t]);
+        lds_order();
+      }
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int i = lane + 64 * t;
+        xv[t] = fmaf(mu * sc[t], dv[t], xv[t]);
+        if (i < D) xs[i] = i < f ? xv[t] : 0.0f;
+      }
+      lds_order();
+    }
+  }
+  if (trace && lane == 0) {
+    trace[0] = rel0;
+    trace[1] = rel;
+    trace[2] = (float)rounds_run;
+    trace[3] = accepted ? 0.0f : 1.0f;
+  }
+  if (!accepted) return false;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int i = lane + 64 * t;
+    const float xo = i < f ? xv[t] : 0.0f;
+    if (i < ldf) xrow[i] = xo;
+    if (xb && i < ldb) xb[i] = (__bf16)xo;
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+template <int NB, bool BF>
+__device__ __forceinline__ void gram_any(f4 (&acc)[num_tiles(NB)], const SweepArgs& a, int64_t start, int count,
+                                         float* lds, int lane) {
+  if constexpr (BF)
+    gram_range_bf16<NB>(acc, a.col, a.val, reinterpret_cast<const __bf16*>(a.theta_b), a.ldb, a.zero16, a.f,
+                        start, count, reinterpret_cast<__bf16*>(lds), lane);
+  else
+    gram_range<NB>(acc, a.col, a.val, a.theta, a.ldf, start, count, lds, lane);
+}
+
+// Solves the system in registers for one row: Cholesky, or (CG) the
+// mixed-precision refinement; a row CG does not accept is appended to the
+// fallback list as `entry` (row >= 0, or -1 - index of a split row).
+template <int NB, bool CG>
+__device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepArgs& a, bool bf, float* lds,
+                                          int64_t row, __bf16* xb, int lane, int entry) {
+  float* xrow = a.x + row * a.ldf;
+  const int bcol = bf ? a.f : (int)a.ldf;
+  const int bcol2 = bf ? a.f + 1 : -1;
+  if constexpr (CG) {
+    const CgParams cp{a.cg_iters, a.cg_rounds, a.cg_tol, a.warm};
+    const float* x0row = a.x0 ? a.x0 + row * a.ldf : nullptr;
+    float* trace = a.cg_trace ? a.cg_trace + 4 * row : nullptr;
+    const bool accepted = cg_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb, cp, x0row, trace);
+    if (!accepted && lane == 0) {
+      a.fb_list[atomicAdd(a.fb_count, 1)] = entry;
+      atomicAdd(a.cg_fallback, 1);
+    }
+  } else {
+    const bool ok = chol_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb);
+    if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
+  }
+}
+
+
+}  // namespace cumf

@@ -64,6 +64,10 @@ struct SweepArgs {
   // -1 - index into heavy for a split row; fb_count = list length.
   int* fb_list;
   int* fb_count;
+  // persistent > 0: launch that many wavefronts, each taking tasks from
+  // *task_counter (zero before the launch) until ntasks are taken.
+  int persistent;
+  int* task_counter;
 };
 
 // The reference-reproducing path (compat.hip): one work-group per row.
@@ -86,10 +90,17 @@ hipError_t launch_compat(const CompatArgs& a, int64_t nrows, hipStream_t s);
 int sweep_nb(int f);
 size_t partial_floats_per_slot(int f);
 int dump_dim(int f);
-hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s);
-hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s);
-hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s);
-hipError_t launch_fallback_reset(int* count, hipStream_t s);
+// Clears n <= 64 ints (CG fallback-list length, persistent task counters).
+hipError_t launch_fallback_reset(int* count, int n, hipStream_t s);
+// Half-sweep kernels (als_sweep.hip), one set per register budget:
+// w2 = two wavefronts per SIMD, w1 = one.
+#define CUMF_SWEEP_LAUNCHERS                                                \
+  hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s);         \
+  hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s);       \
+  hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s);
+namespace w1 { CUMF_SWEEP_LAUNCHERS }
+namespace w2 { CUMF_SWEEP_LAUNCHERS }
+#undef CUMF_SWEEP_LAUNCHERS
 hipError_t launch_dump_kernel(const SweepArgs& a, const int32_t* rows, int64_t nrows, float* out,
                               hipStream_t s);
 hipError_t launch_rmse(const int32_t* u, const int32_t* v, const float* r, int64_t cnt, const float* X,

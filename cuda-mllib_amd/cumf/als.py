@@ -95,6 +95,9 @@ def lib():
         "cumf_debug_cg_trace": ([P, C.c_int, P], C.c_int),
         "cumf_debug_features": ([P, C.c_int, P, C.c_int64, P], C.c_int),
         "cumf_set_mlp": ([P, P], C.c_int),
+        "cumf_load_tile_table": ([P, C.c_char_p], C.c_int),
+        "cumf_save_tile_table": ([P, C.c_char_p], C.c_int),
+        "cumf_autotune": ([P, C.c_int, C.c_char_p], C.c_int),
         "cumf_mlp_create": ([C.c_int, C.c_int, C.c_uint64, C.POINTER(P)], C.c_int),
         "cumf_mlp_destroy": ([P], C.c_int),
         "cumf_mlp_dims": ([P, P, C.POINTER(C.c_int64)], C.c_int),
@@ -264,6 +267,18 @@ class ALS:
         _check(lib().cumf_debug_features(self._h, side, _ptr(rows), rows.shape[0], _ptr(out)),
                "cumf_debug_features")
         return out
+
+    def autotune(self, reps=3, path=None):
+        """Times every degree bucket's launch shapes and keeps the fastest;
+        writes the tile table to path if given."""
+        _check(lib().cumf_autotune(self._h, int(reps), None if path is None else str(path).encode()),
+               "cumf_autotune")
+
+    def load_tile_table(self, path):
+        _check(lib().cumf_load_tile_table(self._h, str(path).encode()), "cumf_load_tile_table")
+
+    def save_tile_table(self, path):
+        _check(lib().cumf_save_tile_table(self._h, str(path).encode()), "cumf_save_tile_table")
 
     def set_mlp(self, mlp):
         _check(lib().cumf_set_mlp(self._h, None if mlp is None else mlp._h), "cumf_set_mlp")

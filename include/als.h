@@ -126,8 +126,8 @@ typedef struct cumf_stats {
   int64_t gram_flops;      /* nnz f(f+1) (the f(f+1)/2 unique Gram entries, one
                               multiply-add each, per rating) summed over the
                               half-sweeps run */
-  double iter_ms;          /* graph replays of whole iterations (use_graph) */
-  int64_t iters;
+  double iter_ms;          /* graph replays (use_graph), one per half-sweep */
+  int64_t iters;           /* number of graph replays */
   int64_t cg_fallback_rows; /* CUMF_SOLVER_CG: rows solved again by Cholesky */
   double warm_ms;          /* features + network predictions (CUMF_WARM_MLP/BEST) */
   int64_t warm_launches;
@@ -195,6 +195,26 @@ int cumf_save_csr_bin(const char* dataFile, const char* rowFile, const char* col
 int cumf_csr_to_csc_device(hipStream_t stream, int64_t m, int64_t n, int64_t nnz,
                            const int32_t* d_row_ptr, const int32_t* d_col, const float* d_val,
                            int32_t* d_col_ptr, int32_t* d_row, float* d_val_out);
+
+/* ------------------------------------------------------------------------
+ * Batching form: degree buckets and the tile table (no counterpart in the
+ * reference, which launches one block per row).
+ *
+ * Tasks (rows, or split chunks of long rows) fall into 8 degree buckets by
+ * rating count ([0,16), [16,64), ..., [16384,65536), [65536,...)).  A tile
+ * table gives, per side (0 = X, 1 = Theta): the split size, the register
+ * budget (1 or 2 wavefronts per SIMD) of the split-row finish and of the CG
+ * fallback kernels, and per bucket the register budget and whether the
+ * bucket runs as a persistent grid of N wavefronts taking tasks from a
+ * counter.  Without a table every half-sweep is one launch of all tasks.
+ * ---------------------------------------------------------------------- */
+int cumf_load_tile_table(cumf_handle* h, const char* path);
+int cumf_save_tile_table(cumf_handle* h, const char* path);
+/* Autotuner: on the loaded ratings, times each split size per side, then
+ * every launch shape of every non-empty bucket (min over reps runs), and
+ * keeps the fastest; the factors are restored afterwards.  Writes the table
+ * to path unless NULL. */
+int cumf_autotune(cumf_handle* h, int reps, const char* path);
 
 /* ------------------------------------------------------------------------
  * Test and measurement hooks (no counterpart in the reference).

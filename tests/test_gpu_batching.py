@@ -1,0 +1,122 @@
+"""GPU checks of the batching form: degree buckets launched with per-bucket
+shapes from a tile table (register budget, persistent grid), the
+autotuner, and per-half-sweep graph replay.  Each configuration must give
+the factors of the single-launch path (same arithmetic: rows are solved
+by the same code whatever launch carries them) to 1e-6 relative."""
+import numpy as np
+import pytest
+
+import cumf
+from cumf import als as G
+from tests import _data
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(ds, f, lam, T0, iters=2, table=None, tune=False, **opt):
+    h = G.ALS(f, lam, **{"timing": 1, **opt})
+    h.set_dataset(ds)
+    h.set_factors(theta=T0)
+    if table is not None:
+        h.load_tile_table(table)
+    if tune:
+        h.autotune(reps=1, path=tune)
+    h.iterate(iters)
+    X, T = h.get_factors()
+    st = h.stats()
+    h.close()
+    return X, T, st
+
+
+def _rel(a, b):
+    return float(np.max(np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-30)))
+
+
+def _table(tmp_path, split, variant_of, persistent_of):
+    lines = ["# test table"]
+    for side in (0, 1):
+        lines.append(f"side {side} split {split} finish {1 + side} fallback {2 - side}")
+        for b in range(8):
+            lines.append(f"bucket {side} {b} 0 0 variant {variant_of(side, b)} persistent {persistent_of(side, b)}")
+    p = tmp_path / "tiles.txt"
+    p.write_text("\n".join(lines) + "\n")
+    return p
+
+
+@pytest.mark.parametrize("solver,precision", [(G.CUMF_SOLVER_CHOLESKY, G.CUMF_PREC_FP32),
+                                              (G.CUMF_SOLVER_CG, G.CUMF_PREC_BF16_GRAM)])
+def test_bucket_shapes_match_single_launch(tmp_path, solver, precision):
+    ds = _data.netflix_like_small()
+    f, lam = 100, 0.058
+    T0 = cumf.init_factors(ds.n, f, 41)
+    Xr, Tr, _ = _run(ds, f, lam, T0, split=256, solver=solver, precision=precision)
+    tab = _table(tmp_path, 256, lambda s, b: 1 + (b + s) % 2, lambda s, b: 0 if b % 3 else 96)
+    Xb, Tb, st = _run(ds, f, lam, T0, table=tab, split=2048, solver=solver, precision=precision)
+    assert st["failed_rows"] == 0
+    assert _rel(Xb, Xr) <= 1e-6 and _rel(Tb, Tr) <= 1e-6, (_rel(Xb, Xr), _rel(Tb, Tr))
+    # one launch per non-empty bucket and half-sweep
+    assert st["task_launches"] > 4
+
+
+def test_all_persistent_buckets_and_graph_replay(tmp_path):
+    ds = _data.netflix_like_small()
+    f, lam = 32, 0.058
+    T0 = cumf.init_factors(ds.n, f, 42)
+    Xr, Tr, _ = _run(ds, f, lam, T0, iters=3, split=512, solver=G.CUMF_SOLVER_CG)
+    tab = _table(tmp_path, 512, lambda s, b: 2 - b % 2, lambda s, b: 64 + 32 * b)
+    Xe, Te, _ = _run(ds, f, lam, T0, iters=3, table=tab, solver=G.CUMF_SOLVER_CG)
+    Xg, Tg, st = _run(ds, f, lam, T0, iters=3, table=tab, solver=G.CUMF_SOLVER_CG, use_graph=1)
+    assert _rel(Xe, Xr) <= 1e-6 and _rel(Te, Tr) <= 1e-6
+    np.testing.assert_array_equal(Xg, Xe)
+    np.testing.assert_array_equal(Tg, Te)
+    assert st["iters"] == 6  # one graph replay per half-sweep
+
+
+def test_autotune_writes_table_and_keeps_factors(tmp_path):
+    ds = _data.netflix_like_small()
+    f, lam = 32, 0.058
+    T0 = cumf.init_factors(ds.n, f, 43)
+    path = tmp_path / "tuned.txt"
+    h = G.ALS(f, lam, solver=G.CUMF_SOLVER_CG, precision=G.CUMF_PREC_BF16_GRAM)
+    h.set_dataset(ds)
+    h.set_factors(theta=T0)
+    h.iterate(1)
+    X1, T1 = h.get_factors()
+    h.autotune(reps=1, path=str(path))
+    X2, T2 = h.get_factors()
+    np.testing.assert_array_equal(X1, X2)
+    np.testing.assert_array_equal(T1, T2)
+    h.iterate(1)
+    Xt, Tt = h.get_factors()
+    h.close()
+    text = path.read_text().splitlines()
+    assert sum(l.startswith("side ") for l in text) == 2
+    assert sum(l.startswith("bucket ") for l in text) == 16
+    # the tuned shapes give the factors of the single-launch path
+    h = G.ALS(f, lam, solver=G.CUMF_SOLVER_CG, precision=G.CUMF_PREC_BF16_GRAM)
+    h.set_dataset(ds)
+    h.set_factors(X=X1, theta=T1)
+    h.load_tile_table(str(path))
+    h.save_tile_table(str(tmp_path / "again.txt"))
+    assert (tmp_path / "again.txt").read_text() == path.read_text()
+    h.close()
+    splits = {int(l.split()[3]) for l in text if l.startswith("side ")}
+    h = G.ALS(f, lam, solver=G.CUMF_SOLVER_CG, precision=G.CUMF_PREC_BF16_GRAM, split=min(splits))
+    h.set_dataset(ds)
+    h.set_factors(X=X1, theta=T1)
+    h.iterate(1)
+    Xs, Ts = h.get_factors()
+    h.close()
+    if len(splits) == 1:
+        assert _rel(Xt, Xs) <= 1e-6 and _rel(Tt, Ts) <= 1e-6
+
+
+def test_bad_tile_table_rejected(tmp_path):
+    p = tmp_path / "bad.txt"
+    p.write_text("side 0 split 0 finish 2 fallback 2\n")
+    with G.ALS(16, 0.05) as h:
+        with pytest.raises(G.CumfError):
+            h.load_tile_table(str(p))
+        p.write_text("bucket 0 9 0 0 variant 2 persistent 0\n")
+        with pytest.raises(G.CumfError):
+            h.load_tile_table(str(p))
