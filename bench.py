@@ -96,6 +96,25 @@ def measured_traffic(workload, precision, solver, f):
     return None
 
 
+def precision_report(ds, f, lam, theta, X, nrows):
+    """Error of one X half-sweep against the FP64 oracle on sampled rows:
+    relative residual ||b - A x|| / ||b|| and factor error
+    ||x - x*|| / ||x*|| with A, b, x* formed in FP64 from the same theta."""
+    from oracle import oracle as O
+    rows = np.linspace(0, ds.m - 1, nrows).astype(np.int64)
+    res, err = [], []
+    for u in rows:
+        if ds.csr_row_ptr[u + 1] == ds.csr_row_ptr[u]:
+            continue
+        A, b = O.row_system(ds.csr_row_ptr, ds.csr_col, ds.csr_val, theta, f, lam, int(u))
+        x = X[u].astype(np.float64)
+        xs = np.linalg.solve(A, b)
+        res.append(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+        err.append(np.linalg.norm(x - xs) / np.linalg.norm(xs))
+    q = lambda v: {"median": float(np.median(v)), "p99": float(np.percentile(v, 99)), "max": float(np.max(v))}
+    return {"rows": len(res), "relative_residual": q(res), "factor_rel_error": q(err)}
+
+
 def train_network(G, ds, f, lam, args, prec):
     """Warm-start network trained on pairs the plain solver produces from
     another random start on the same ratings (not timed)."""
@@ -151,16 +170,20 @@ def run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net=None, tune=Tru
     X, T = h.get_factors()
     # Per-half-sweep split, after the timed region (one more iteration).
     sweep_ms = {}
+    X_next = None
     for name, fn in (("x", h.update_x), ("theta", h.update_theta)):
         h.synchronize()
         t1 = time.perf_counter()
         fn()
         h.synchronize()
         sweep_ms[name] = (time.perf_counter() - t1) * 1e3
+        if name == "x" and args.precision_report:
+            X_next = h.get_factors()[0]
     opt = h.opt
     h.close()
+    report = precision_report(ds, f, lam, T, X_next, args.precision_report) if X_next is not None else None
     return {"s": elapsed / args.steps, "rmse": rmse, "stats": st, "sweep_ms": sweep_ms, "X": X, "T": T,
-            "opt": opt, "tuning": tuning}
+            "opt": opt, "tuning": tuning, "precision_report": report}
 
 
 def main():
@@ -187,6 +210,8 @@ def main():
                     help="tune the degree-bucket launch shapes on this workload first (untimed)")
     ap.add_argument("--tile-table", default="", help="load degree-bucket launch shapes from this file")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
+    ap.add_argument("--precision-report", type=int, default=0, metavar="ROWS",
+                    help="per-row FP64 residual and factor error of one more X half-sweep on ROWS sampled rows")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -297,9 +322,13 @@ def main():
                             "achieved": gather_bytes_launch / (task_ms * 1e-3) / 1e9,
                             "frac": gather_bytes_launch / (task_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
     }
+    if res["precision_report"] is not None:
+        rec["precision_report"] = res["precision_report"]
     if plain is not None:
         rec["plain_fp32"] = {"s_per_iteration": plain["s"], "heldout_rmse": plain["rmse"],
                              "half_sweep_ms": plain["sweep_ms"]}
+        if plain["precision_report"] is not None:
+            rec["plain_fp32"]["precision_report"] = plain["precision_report"]
         rec["speedup_vs_plain_fp32"] = plain["s"] / s_per_iter
         rec["rmse_delta_vs_plain_fp32"] = res["rmse"] - plain["rmse"]
     if not args.no_cpu_baseline:

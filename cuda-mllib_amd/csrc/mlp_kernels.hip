@@ -215,69 +215,67 @@ __global__ __launch_bounds__(256) void mlp_to_bf16_kernel(const float* __restric
 }
 
 // Features of rows row0 .. row0+nrows-1 of a CSR side, one wavefront per
-// row: lane l accumulates columns l, l+64, l+128 of diag(Y^T Y) and Y^T r
-// in FP32 over the row's ratings in order; the row of K0 BF16 values is
-// [log n_u, diag/n_u + lambda, b/n_u, 0...] (all zero for an empty row).
+// row: [log n_u, diag/n_u + lambda, b/n_u, 0...] (all zero for an empty
+// row) in BF16, from FP32 sums diag_i = sum theta_vi^2, b_i = sum r theta_vi.
+// The wavefront works as four groups of 16 lanes; group q takes ratings
+// q, q+4, q+8, ... of the row, lane c of a group holds columns 4c..4c+3
+// (+64, +128) and moves them with 16-byte loads; the four group sums are
+// added in group order at the end.
 __global__ __launch_bounds__(64) void mlp_features_kernel(const int32_t* __restrict__ row_ptr,
                                                           const int32_t* __restrict__ col,
                                                           const float* __restrict__ val,
                                                           const float* __restrict__ theta, int64_t ldf, int f,
                                                           float lambda, int64_t row0, int K0,
                                                           __bf16* __restrict__ out) {
-  const int lane = threadIdx.x;
+  __shared__ float sd[4][192], sb[4][192];
+  const int lane = threadIdx.x, q = lane >> 4, c = lane & 15;
   const int64_t u = row0 + blockIdx.x;
   const int64_t s = row_ptr[u], e = row_ptr[u + 1];
-  float d[3] = {0.f, 0.f, 0.f}, b[3] = {0.f, 0.f, 0.f};
-  int64_t k = s;
-  for (; k + 4 <= e; k += 4) {
-    int v[4];
-    float r[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      v[q] = col[k + q];
-      r[q] = val[k + q];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int t = 0; t < 3; ++t) {
-        const int i = lane + 64 * t;
-        if (i < f) {
-          const float th = theta[(int64_t)v[q] * ldf + i];
-          d[t] = fmaf(th, th, d[t]);
-          b[t] = fmaf(r[q], th, b[t]);
-        }
-      }
-  }
-  for (; k < e; ++k) {
-    const int v = col[k];
-    const float r = val[k];
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      const int i = lane + 64 * t;
-      if (i < f) {
-        const float th = theta[(int64_t)v * ldf + i];
-        d[t] = fmaf(th, th, d[t]);
-        b[t] = fmaf(r, th, b[t]);
-      }
-    }
-  }
-  const int n = (int)(e - s);
-  const float inv = n > 0 ? 1.0f / (float)n : 0.0f;
-  __shared__ float sd[192], sb[192];
+  const int nq = (int)((ldf + 63) / 64);  // 64-column slices of a factor row (<= 3)
+  f4 d[3], b[3];
 #pragma unroll
   for (int t = 0; t < 3; ++t) {
-    sd[lane + 64 * t] = d[t] * inv + lambda;
-    sb[lane + 64 * t] = b[t] * inv;
+    d[t] = f4{0.f, 0.f, 0.f, 0.f};
+    b[t] = f4{0.f, 0.f, 0.f, 0.f};
   }
+  for (int64_t k = s + q; k < e; k += 4) {
+    const float r = val[k];
+    const f4* th = reinterpret_cast<const f4*>(theta + (int64_t)col[k] * ldf);
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      if (t < nq && 64 * t + 4 * c < ldf) {
+        const f4 v = th[16 * t + c];
+#pragma unroll
+        for (int z = 0; z < 4; ++z) {
+          d[t][z] = fmaf(v[z], v[z], d[t][z]);
+          b[t][z] = fmaf(r, v[z], b[t][z]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 3; ++t)
+#pragma unroll
+    for (int z = 0; z < 4; ++z) {
+      sd[q][64 * t + 4 * c + z] = d[t][z];
+      sb[q][64 * t + 4 * c + z] = b[t][z];
+    }
   __syncthreads();
+  const int n = (int)(e - s);
+  const float inv = n > 0 ? 1.0f / (float)n : 0.0f;
   __bf16* o = out + (int64_t)blockIdx.x * K0;
   for (int i = lane; i < K0; i += 64) {
     float vv = 0.0f;
     if (n > 0) {
-      if (i == 0) vv = logf((float)n);
-      else if (i <= f) vv = sd[i - 1];
-      else if (i <= 2 * f) vv = sb[i - 1 - f];
+      if (i == 0) {
+        vv = logf((float)n);
+      } else if (i <= f) {
+        const int j = i - 1;
+        vv = (((sd[0][j] + sd[1][j]) + sd[2][j]) + sd[3][j]) * inv + lambda;
+      } else if (i <= 2 * f) {
+        const int j = i - 1 - f;
+        vv = (((sb[0][j] + sb[1][j]) + sb[2][j]) + sb[3][j]) * inv;
+      }
     }
     o[i] = (__bf16)vv;
   }
@@ -337,7 +335,7 @@ hipError_t launch_mlp_features(const int32_t* row_ptr, const int32_t* col, const
                                int64_t ldf, int f, float lambda, int64_t row0, int64_t nrows, int K0, __bf16* out,
                                hipStream_t s) {
   if (nrows <= 0) return hipSuccess;
-  if (f > 192) return hipErrorInvalidValue;
+  if (f > 192 || ldf > 192 || ldf % 4) return hipErrorInvalidValue;
   hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)nrows), dim3(64), 0, s, row_ptr, col, val, theta, ldf, f,
                      lambda, row0, K0, out);
   return hipGetLastError();

@@ -69,7 +69,7 @@ def _check_sweep(ds_rp, ds_ci, ds_cv, theta, Xg, f, lam, rows):
     return rel
 
 
-@pytest.mark.parametrize("f,split", [(32, 2048), (100, 2048), (100, 64), (10, 200)])
+@pytest.mark.parametrize("f,split", [(32, 2048), (100, 2048), (100, 64), (10, 200), (128, 300)])
 def test_half_sweeps_match_oracle(f, split):
     ds = _data.netflix_like_small()
     lam = 0.058
