@@ -76,12 +76,12 @@ struct Plan {
         return CUMF_ERR_INVALID;
       }
       if (n <= split) {
-        t.push_back({(int32_t)u, 0, (int32_t)n, -1});
+        t.push_back({(int32_t)u, row_ptr[u], (int32_t)n, -1});
       } else {
         const int64_t nc = (n + split - 1) / split;
         hv.push_back({(int32_t)u, (int32_t)slot, (int32_t)nc, 0});
         for (int64_t c = 0; c < nc; ++c)
-          t.push_back({(int32_t)u, (int32_t)(c * split), (int32_t)std::min<int64_t>(split, n - c * split),
+          t.push_back({(int32_t)u, (int32_t)(row_ptr[u] + c * split), (int32_t)std::min<int64_t>(split, n - c * split),
                        (int32_t)(slot + c)});
         slot += nc;
       }

@@ -51,7 +51,7 @@ __host__ __device__ constexpr int tile_index(int I, int J, int NB) {
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ constexpr int lds_floats_nb(int NB, int f) {
   return cmax(cmax(2 * KB * ((f + 3) & ~3) + 2 * KB,   // FP32 staging
-                   2 * 32 * 16 * NB / 2),              // BF16 staging (2 x 32 rows x 16NB bf16)
+                   3 * 32 * 16 * NB / 2),              // BF16 staging (3 x 32 rows x 16NB bf16)
               cmax(f * (f + 1) / 2 + 2 * f, 32 * NB));  // U, z, z2; two pivot-row buffers
 }
 
@@ -66,6 +66,13 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 // including LDS-DMA loads (global_load_lds), whose completion the compiler
 // does not track; also a compiler memory barrier.
 __device__ __forceinline__ void wait_vm_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Waits until at most N vector-memory operations of this wave are
+// outstanding (they complete in issue order), also a compiler barrier.
+template <int N>
+__device__ __forceinline__ void wait_vm_recent() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
 // Orders LDS accesses of one wave (LDS executes a wave's DS instructions in
 // issue order, so only the compiler must be kept from reordering).
 __device__ __forceinline__ void lds_order() {
@@ -201,7 +208,7 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
     return (lane < KBB && ch < nch && k < count) ? val[start + k] : 0.0f;
   };
   auto issue = [&](int ch, int ci) {
-    __bf16* yb = lds + (ch & 1) * CB;
+    __bf16* yb = lds + (ch % 3) * CB;
 #pragma unroll
     for (int it = 0; it < NB; ++it) {
       const int sl = it * 64 + lane;  // NB*64 = KBB*SPR slots exactly
@@ -213,24 +220,36 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
     }
   };
 
-  int ci = load_ci(0);
-  float rv = load_r(0);
-  issue(0, ci);
-  ci = load_ci(1);
-  float rnext = load_r(1);
+  // Gathers run two chunks ahead (three LDS buffers); column indices and
+  // ratings three chunks ahead, in registers.
+  int ci1 = load_ci(1), ci2 = load_ci(2);
+  float rv = load_r(0), r1 = load_r(1), r2 = load_r(2);
+  issue(0, load_ci(0));
+  if (1 < nch) issue(1, ci1);
+  int cn = load_ci(3);
+  float rn = load_r(3);
   for (int ch = 0; ch < nch; ++ch) {
-    wait_vm_all();  // chunk ch staged; indices and ratings of ch+1 loaded
-    __bf16* yb = lds + (ch & 1) * CB;
+    // Chunk ch's gathers are older than the last NB vector-memory
+    // operations whenever the two following chunks were issued (the next
+    // chunk's NB gathers plus the index/rating loads came after them);
+    // near the end of the range wait for everything.
+    if (ch + 2 < nch) wait_vm_recent<NB>();
+    else wait_vm_all();
+    __bf16* yb = lds + (ch % 3) * CB;
     if (lane < KBB && ch * KBB + lane < count) {
       const __bf16 hi = (__bf16)rv;
       const __bf16 lo = (__bf16)(rv - (float)hi);
       yb[lane * W + f] = hi;
       yb[lane * W + f + 1] = lo;
     }
-    if (ch + 1 < nch) issue(ch + 1, ci);
-    rv = rnext;
-    ci = load_ci(ch + 2);
-    rnext = load_r(ch + 2);
+    if (ch + 2 < nch) issue(ch + 2, ci2);
+    asm volatile("" ::: "memory");  // keep the next loads after the gathers
+    rv = r1;
+    r1 = r2;
+    r2 = rn;
+    ci2 = cn;
+    cn = load_ci(ch + 4);
+    rn = load_r(ch + 4);
     lds_order();
     bf8 op[NB];
 #pragma unroll
@@ -494,10 +513,20 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
 }
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+// FP64 sum over the wave, uniform, in a fixed order (as wave_sum).
 __device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return readlane_d(v, 0);
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x124>(v);
+  v += dpp_d<0x128>(v);
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
 // A register value the compiler must treat as changed at this point: keeps
@@ -621,11 +650,14 @@ __device__ __forceinline__ double residual_norm2(const f4 (&acc)[num_tiles(NB)],
 // the row to the Cholesky fallback kernel.
 // x0row: the network's prediction for this row (warm 2 and 3).  trace
 // (optional): initial and final relative residual, rounds run, fell back.
+// xpre (optional): this lane's entries of the current factor, already
+// loaded by the caller.
 template <int NB>
 __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bcol, float* __restrict__ lds,
                                          float* __restrict__ xrow, int64_t ldf, int lane, int bcol2,
                                          __bf16* __restrict__ xb, int64_t ldb, const CgParams& cp,
-                                         const float* __restrict__ x0row, float* __restrict__ trace) {
+                                         const float* __restrict__ x0row, float* __restrict__ trace,
+                                         const float* xpre = nullptr) {
   constexpr int RPL = rows_per_lane(NB);
   constexpr int D = 16 * NB;
   const int g = lane >> 4, c = lane & 15;
@@ -639,20 +671,26 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
   float* bs = reinterpret_cast<float*>(ys);        // b (and the BF16 low part at bs + D)
   float* ss = reinterpret_cast<float*>(ys);        // Jacobi scales, after b is read
 
-  // Diagonal and right-hand side out of the register tiles.
+  // Diagonal and right-hand side out of the register tiles.  Lane (g, c)
+  // of diagonal tile I holds a_ii for i = 16I + c when c >> 2 == g (its
+  // register c & 3); the right-hand side is the column bcol (bcol2) of the
+  // block-column bcol >> 4, held by the lanes with c == bcol & 15.  One
+  // store per lane and block (rows past f are written too and never read).
   lds_order();
+  const int jb = bcol >> 4, cb = bcol & 15;
+  const int jb2 = bcol2 >= 0 ? bcol2 >> 4 : -1, cb2 = bcol2 & 15;
 #pragma unroll
-  for (int I = 0; I < NB; ++I)
+  for (int I = 0; I < NB; ++I) {
+    const f4 dv = acc[tile_index(I, I, NB)];
+    const int rr = c & 3;
+    const float dval = rr == 0 ? dv[0] : (rr == 1 ? dv[1] : (rr == 2 ? dv[2] : dv[3]));
+    if ((c >> 2) == g) ds[16 * I + c] = dval;
 #pragma unroll
-    for (int J = I; J < NB; ++J)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * I + 4 * g + r, j = 16 * J + c;
-        const float v = acc[tile_index(I, J, NB)][r];
-        if (I == J && i == j && i < f) ds[i] = v;
-        if (j == bcol && i < f) bs[i] = v;
-        if (j == bcol2 && i < f) bs[D + i] = v;
-      }
+    for (int J = I; J < NB; ++J) {
+      if (J == jb && c == cb) *reinterpret_cast<f4*>(bs + 16 * I + 4 * g) = acc[tile_index(I, J, NB)];
+      if (J == jb2 && c == cb2) *reinterpret_cast<f4*>(bs + D + 16 * I + 4 * g) = acc[tile_index(I, J, NB)];
+    }
+  }
   lds_order();
   float sc[RPL], bv[RPL], xv[RPL], xm[RPL];
   bool bad = false, mbad = false;
@@ -665,7 +703,7 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
     bad = bad || !good;
     sc[t] = (in && good) ? __builtin_amdgcn_rsqf(d) : 0.0f;
     bv[t] = in ? bs[i] + (bcol2 >= 0 ? bs[D + i] : 0.0f) : 0.0f;
-    float x0 = (in && cp.warm != 1) ? xrow[i] : 0.0f;
+    float x0 = (in && cp.warm != 1) ? (xpre ? xpre[t] : xrow[i]) : 0.0f;
     if (!(fabsf(x0) < __builtin_huge_valf())) x0 = 0.0f;
     xv[t] = x0;
     xm[t] = (in && cp.warm >= 2) ? x0row[i] : 0.0f;
@@ -695,9 +733,13 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
   if (!any_bad) {
     // A^ in FP16: tile (I, J) writes rows 16I+4g+r at column 16J+c and, for
     // I < J, row 16J+c at columns 16I+4g..+3 (one 8-byte store).
+    // Blocks that lie wholly inside the f x f8 window (all but the last
+    // row / column block) store without per-lane predicates; the test is
+    // uniform across the wavefront.
 #pragma unroll
     for (int I = 0; I < NB; ++I) {
       const f4 sr = *reinterpret_cast<const f4*>(ss + 16 * I + 4 * g);
+      const bool rows_in = 16 * I + 16 <= f;
 #pragma unroll
       for (int J = I; J < NB; ++J) {
         const int j = 16 * J + c;
@@ -706,12 +748,20 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
         h4 tv;
 #pragma unroll
         for (int r = 0; r < 4; ++r) tv[r] = (_Float16)(v[r] * sr[r] * scj);
-        if (j < f8) {
+        _Float16* drow = Ah + (16 * I + 4 * g) * sa + j;
+        if (rows_in && 16 * J + 16 <= f8) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) drow[r * sa] = tv[r];
+        } else if (j < f8) {
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            if (16 * I + 4 * g + r < f) Ah[(16 * I + 4 * g + r) * sa + j] = tv[r];
+            if (16 * I + 4 * g + r < f) drow[r * sa] = tv[r];
         }
-        if (I < J && j < f && 16 * I + 4 * g < f8) *reinterpret_cast<h4*>(Ah + j * sa + 16 * I + 4 * g) = tv;
+        if (I < J) {
+          h4* dt = reinterpret_cast<h4*>(Ah + j * sa + 16 * I + 4 * g);
+          if (16 * J + 16 <= f && 16 * I + 16 <= f8) *dt = tv;
+          else if (j < f && 16 * I + 4 * g < f8) *dt = tv;
+        }
       }
     }
     double bb = 0.0;
@@ -848,7 +898,8 @@ __device__ __forceinline__ void gram_any(f4 (&acc)[num_tiles(NB)], const SweepAr
 // fallback list as `entry` (row >= 0, or -1 - index of a split row).
 template <int NB, bool CG>
 __device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepArgs& a, bool bf, float* lds,
-                                          int64_t row, __bf16* xb, int lane, int entry) {
+                                          int64_t row, __bf16* xb, int lane, int entry,
+                                          const float* xpre = nullptr) {
   float* xrow = a.x + row * a.ldf;
   const int bcol = bf ? a.f : (int)a.ldf;
   const int bcol2 = bf ? a.f + 1 : -1;
@@ -856,7 +907,8 @@ __device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepA
     const CgParams cp{a.cg_iters, a.cg_rounds, a.cg_tol, a.warm};
     const float* x0row = a.x0 ? a.x0 + row * a.ldf : nullptr;
     float* trace = a.cg_trace ? a.cg_trace + 4 * row : nullptr;
-    const bool accepted = cg_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb, cp, x0row, trace);
+    const bool accepted =
+        cg_solve<NB>(acc, a.f, bcol, lds, xrow, a.ldf, lane, bcol2, xb, a.ldb, cp, x0row, trace, xpre);
     if (!accepted && lane == 0) {
       a.fb_list[atomicAdd(a.fb_count, 1)] = entry;
       atomicAdd(a.cg_fallback, 1);

@@ -7,7 +7,8 @@
 namespace cumf {
 
 // One unit of work for a wavefront: `count` ratings of row `row`, starting
-// `start` ratings after the row's first.  slot < 0: the task covers the
+// at rating index `start` of the CSR (absolute, so that no row-pointer load
+// precedes the first gather; rating indices fit int32 like the CSR's own).  slot < 0: the task covers the
 // whole row and solves it; slot >= 0: it writes a partial Gram to
 // partial[slot].
 struct Task {

@@ -34,8 +34,18 @@ __device__ __forceinline__ void run_task(const SweepArgs& a, const Task& t, floa
 #pragma unroll
   for (int i = 0; i < num_tiles(NB); ++i) acc[i] = f4{0.f, 0.f, 0.f, 0.f};
 
-  const int64_t start = (int64_t)a.row_ptr[t.row] + t.start;
-  if (t.count > 0) gram_any<NB, BF>(acc, a, start, t.count, lds, lane);
+  // The row's current factor (the CG start) is fetched before the Gram so
+  // that its latency hides behind the gathers.
+  float xpre[rows_per_lane(NB)];
+  if constexpr (CG) {
+    const float* xr = a.x + (int64_t)t.row * a.ldf;
+#pragma unroll
+    for (int q = 0; q < rows_per_lane(NB); ++q) {
+      const int i = lane + 64 * q;
+      xpre[q] = (t.slot < 0 && i < a.f) ? xr[i] : 0.0f;
+    }
+  }
+  if (t.count > 0) gram_any<NB, BF>(acc, a, (int64_t)t.start, t.count, lds, lane);
 
   if (t.slot >= 0) {  // partial Gram of a heavy row
     float* dst = a.partial + (int64_t)t.slot * num_tiles(NB) * 256;
@@ -55,7 +65,7 @@ __device__ __forceinline__ void run_task(const SweepArgs& a, const Task& t, floa
   }
   add_ridge<NB>(acc, a.f, a.lambda * (float)t.count, lane);
   lds_order();
-  solve_row<NB, CG>(acc, a, BF, lds, t.row, xb, lane, t.row);
+  solve_row<NB, CG>(acc, a, BF, lds, t.row, xb, lane, t.row, CG ? xpre : nullptr);
 }
 
 // tasks[0 .. ntasks): one work-group per task, or (PERSISTENT) a grid of
