@@ -37,7 +37,7 @@ namespace {
 constexpr int kMaxF = 140;
 constexpr int kDefaultSplit = 2048;
 constexpr int kDefaultCgIters = 6;
-constexpr int kDefaultCgRounds = 2;
+constexpr int kDefaultCgRounds = 3;
 constexpr float kDefaultCgTol = 1e-3f;
 
 int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }

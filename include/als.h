@@ -107,7 +107,7 @@ typedef struct cumf_options {
                            takes it from CUMF_COMPAT_HERMITIAN_BUG=1. */
   int32_t solver;       /* CUMF_SOLVER_* */
   int32_t cg_iters;     /* CG steps per refinement round (0: 6) */
-  int32_t cg_rounds;    /* refinement rounds (0: 2) */
+  int32_t cg_rounds;    /* refinement rounds, at most (0: 3) */
   float cg_tol;         /* accept ||b - A x|| <= cg_tol ||b|| (0: 1e-3) */
   int32_t warm_start;   /* CUMF_WARM_* (CG only) */
   int32_t reserved[5];
