@@ -5,6 +5,8 @@
 // held-out RMSE.  This driver takes the same knobs minus the Spark master:
 //
 //   runals gpu <rank> <iterations> <lambda> <dataset> [--split N] [--seed S]
+//          [--precision fp32|bf16] [--solver cholesky|cg] [--graph 0|1]
+//          [--tiles <file>] [--autotune <file>]
 //
 // <dataset> is a generator preset name (see cumf_gen_preset) or
 //   csr:<m>:<n>:<nnz>:<prefix>   raw CSR files <prefix>.row.bin, .col.bin,
@@ -29,7 +31,8 @@ namespace {
 int usage() {
   std::fprintf(stderr,
                "usage: runals <cpu|gpu> <rank> <iterations> <lambda> <dataset> [--split N] [--seed S]\n"
-               "             [--test <count>:<prefix>]\n"
+               "             [--test <count>:<prefix>] [--precision fp32|bf16] [--solver cholesky|cg]\n"
+               "             [--graph 0|1] [--tiles <tile table>] [--autotune <tile table to write>]\n"
                "  dataset: preset name (zipf_10k_2k_500k, netflix_480k_17770_99m, zipf_4m_500k_1b)\n"
                "           or csr:<m>:<n>:<nnz>:<prefix> (raw <prefix>.row.bin/.col.bin/.data.bin)\n");
   return 2;
@@ -61,13 +64,22 @@ int main(int argc, char** argv) {
   const int iters = std::atoi(argv[3]);
   const float lambda = (float)std::atof(argv[4]);
   const std::string dataset = argv[5];
-  int split = 0;
+  int split = 0, precision = CUMF_PREC_FP32, solver = CUMF_SOLVER_CHOLESKY, graph = 0;
   uint64_t seed = 0;
-  std::string test_spec;
+  std::string test_spec, tiles, autotune;
+  if ((argc - 6) % 2) return usage();
   for (int i = 6; i + 1 < argc; i += 2) {
+    const std::string v = argv[i + 1];
     if (!std::strcmp(argv[i], "--split")) split = std::atoi(argv[i + 1]);
     else if (!std::strcmp(argv[i], "--seed")) seed = std::strtoull(argv[i + 1], nullptr, 10);
-    else if (!std::strcmp(argv[i], "--test")) test_spec = argv[i + 1];
+    else if (!std::strcmp(argv[i], "--test")) test_spec = v;
+    else if (!std::strcmp(argv[i], "--precision") && (v == "fp32" || v == "bf16"))
+      precision = v == "bf16" ? CUMF_PREC_BF16_GRAM : CUMF_PREC_FP32;
+    else if (!std::strcmp(argv[i], "--solver") && (v == "cholesky" || v == "cg"))
+      solver = v == "cg" ? CUMF_SOLVER_CG : CUMF_SOLVER_CHOLESKY;
+    else if (!std::strcmp(argv[i], "--graph")) graph = std::atoi(argv[i + 1]) != 0;
+    else if (!std::strcmp(argv[i], "--tiles")) tiles = v;
+    else if (!std::strcmp(argv[i], "--autotune")) autotune = v;
     else return usage();
   }
   if (f <= 0 || iters < 0) return usage();
@@ -127,6 +139,9 @@ int main(int argc, char** argv) {
   cumf_default_options(&opt);
   opt.split = split;
   opt.timing = 1;
+  opt.precision = precision;
+  opt.solver = solver;
+  opt.use_graph = graph;
   cumf_handle* h = nullptr;
   auto check = [&](int rc, const char* what) {
     if (rc != CUMF_OK) {
@@ -143,6 +158,12 @@ int main(int argc, char** argv) {
   std::vector<float> theta((size_t)n * f);
   cumf_gen_init_factors(n, f, seed, theta.data());
   check(cumf_set_factors(h, nullptr, theta.data()), "cumf_set_factors");
+  if (!tiles.empty()) check(cumf_load_tile_table(h, tiles.c_str()), "cumf_load_tile_table");
+  if (!autotune.empty()) {
+    const double t0 = now_s();
+    check(cumf_autotune(h, 2, autotune.c_str()), "cumf_autotune");
+    std::fprintf(stderr, "runals: autotuned in %.1fs, table written to %s\n", now_s() - t0, autotune.c_str());
+  }
 
   double total = 0.0, rmse = 0.0;
   for (int it = 0; it < iters; ++it) {
@@ -157,9 +178,12 @@ int main(int argc, char** argv) {
   cumf_stats st;
   check(cumf_get_stats(h, &st), "cumf_get_stats");
   std::printf("{\"rank\": %d, \"iterations\": %d, \"lambda\": %g, \"m\": %lld, \"n\": %lld, \"nnz\": %lld, "
-              "\"s_per_iteration\": %.6f, \"heldout_rmse\": %.6f, \"failed_rows\": %lld}\n",
+              "\"precision\": \"%s\", \"solver\": \"%s\", "
+              "\"s_per_iteration\": %.6f, \"heldout_rmse\": %.6f, \"failed_rows\": %lld, "
+              "\"cg_fallback_rows\": %lld}\n",
               f, iters, (double)lambda, (long long)m, (long long)n, (long long)nnz,
-              iters ? total / iters : 0.0, rmse, (long long)st.failed_rows);
+              precision == CUMF_PREC_BF16_GRAM ? "bf16" : "fp32", solver == CUMF_SOLVER_CG ? "cg" : "cholesky",
+              iters ? total / iters : 0.0, rmse, (long long)st.failed_rows, (long long)st.cg_fallback_rows);
   cumf_destroy(h);
   return 0;
 }

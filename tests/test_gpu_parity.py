@@ -244,6 +244,24 @@ def test_runals_driver_small_preset():
     assert 0.2 < summary["heldout_rmse"] < 1.0
 
 
+def test_runals_mixed_path_with_autotune(tmp_path):
+    import json
+    import subprocess
+    from cumf import _paths
+    exe = os.path.join(_paths.PKG_DIR, "runals")
+    table = str(tmp_path / "tiles.txt")
+    r = subprocess.run([exe, "gpu", "32", "3", "0.05", "zipf_10k_2k_500k", "--precision", "bf16", "--solver", "cg",
+                        "--autotune", table, "--graph", "1"], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    assert summary["precision"] == "bf16" and summary["solver"] == "cg" and summary["failed_rows"] == 0
+    assert 0.2 < summary["heldout_rmse"] < 1.0
+    r2 = subprocess.run([exe, "gpu", "32", "3", "0.05", "zipf_10k_2k_500k", "--precision", "bf16", "--solver", "cg",
+                         "--tiles", table], capture_output=True, text=True, timeout=180)
+    assert r2.returncode == 0, r2.stderr
+    assert abs(json.loads(r2.stdout.strip().splitlines()[-1])["heldout_rmse"] - summary["heldout_rmse"]) < 1e-6
+
+
 def test_runals_reads_reference_binary_csr(tmp_path):
     ds = _data.tiny()
     import subprocess
