@@ -298,7 +298,7 @@ def main():
                     else " / fp16 cg + fp32 refinement + fp64 residual"),
         "data": "synthetic",
         "config": {"workload": args.workload, "m": ds.m, "n": ds.n, "nnz": ds.nnz, "f": f,
-                   "lambda": lam, "path": path, "graph": bool(args.graph), "split": args.split or 2048,
+                   "lambda": lam, "path": path, "graph": bool(args.graph), "split": args.split or 16384,
                    "iterations_timed": f"{args.warmup + 1}..{args.warmup + args.steps}"},
         "ratings_per_s": ds.nnz / s_per_iter,
         "hermitian_tflops": gram_flops_iter / (sweep_kernel_ms / args.steps * 1e-3) / 1e12,

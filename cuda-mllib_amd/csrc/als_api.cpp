@@ -35,9 +35,9 @@ using cumf::DevBuf;
 namespace {
 
 constexpr int kMaxF = 140;
-constexpr int kDefaultSplit = 2048;
+constexpr int kDefaultSplit = 16384;
 constexpr int kDefaultCgIters = 6;
-constexpr int kDefaultCgRounds = 3;
+constexpr int kDefaultCgRounds = 6;
 constexpr float kDefaultCgTol = 1e-3f;
 
 int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }
@@ -239,6 +239,12 @@ int timed(cumf_handle* h, int kind, F&& launch) {
   return CUMF_OK;
 }
 
+// Register budget of the launches without a tile table: two wavefronts
+// per SIMD while the accumulator tiles leave room for the solver's
+// registers (up to 8 column blocks, f <= 124), one wavefront per SIMD
+// beyond (the two-wavefront build spills several hundred bytes per lane).
+int default_variant(int f) { return cumf::sweep_nb(f) >= 9 ? 1 : 2; }
+
 hipError_t task_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
   return variant == 1 ? cumf::w1::launch_task_kernel(a, s) : cumf::w2::launch_task_kernel(a, s);
 }
@@ -264,7 +270,7 @@ hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::Sw
 // per non-empty degree bucket (heaviest first), persistent ones after their
 // counters are cleared.
 int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs& a) {
-  if (!h->tiled) return timed(h, Timer::TASK, [&] { return task_launch(a, 2, h->stream); });
+  if (!h->tiled) return timed(h, Timer::TASK, [&] { return task_launch(a, default_variant(h->f), h->stream); });
   bool persistent = false;
   for (int b = 0; b < kBuckets; ++b) persistent = persistent || h->tiles[side][b].persistent > 0;
   if (persistent)
@@ -387,7 +393,8 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
   }
   if (cg) CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
   CUMF_TRY(launch_tasks(h, side, p, a));
-  const int fv = h->tiled ? h->finish_variant[side] : 2, bv = h->tiled ? h->fallback_variant[side] : 2;
+  const int dv = default_variant(h->f);
+  const int fv = h->tiled ? h->finish_variant[side] : dv, bv = h->tiled ? h->fallback_variant[side] : dv;
   if (p.nheavy > 0)
     CUMF_TRY(timed(h, Timer::FINISH, [&] { return finish_launch(a, fv, h->stream); }));
   if (cg) CUMF_TRY(timed(h, Timer::FALLBACK, [&] { return fallback_launch(a, bv, h->stream); }));
@@ -949,7 +956,7 @@ int autotune_side(cumf_handle* h, int side, int reps) {
   for (int64_t u = 0; u < rows; ++u) maxdeg = std::max<int64_t>(maxdeg, (int64_t)rp[u + 1] - rp[u]);
   // 1. Split size: whole half-sweeps, one launch per kernel.
   h->tiled = false;
-  static const int kSplits[] = {512, 1024, 2048, 4096, 8192};
+  static const int kSplits[] = {1024, 2048, 4096, 8192, 16384, 32768};
   double best = 1e300;
   int best_split = h->opt.split;
   for (int sp : kSplits) {

@@ -792,20 +792,45 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
     for (int round = 0;; ++round) {
       // FP64 residual of the current x (xs holds it).
       gram_matvec_f64<NB>(acc, xs, ys, lane);
+      // A start far from the solution (the factor of the previous sweep
+      // after the other side changed a lot) is first replaced by its
+      // energy-optimal multiple s x0, s = <x0, b> / <x0, A x0>, whose
+      // A-norm error is never above that of x0 or of zero.  The residual
+      // b - s A x0 then comes from the same product; acceptance waits for
+      // the next round's residual of the rounded s x0.  Starts with
+      // |s - 1| <= 1/8 are kept as they are.
+      double s = 1.0;
+      if (round == 0 && cp.warm != 1) {
+        double xbd = 0.0, xad = 0.0;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int i = lane + 64 * t;
+          if (i < f) {
+            xbd += (double)xv[t] * (double)bv[t];
+            xad += (double)xv[t] * ys[i];
+          }
+        }
+        xbd = wave_sum_d(xbd);
+        xad = wave_sum_d(xad);
+        const double so = xbd / xad;
+        if (xad > 0.0 && !(fabs(so - 1.0) <= 0.125)) s = so;
+      }
+      const bool scaled = s != 1.0;
       float rh[RPL];
       double rr = 0.0;
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
         const int i = lane + 64 * t;
-        const double r = i < f ? (double)bv[t] - ys[i] : 0.0;
+        const double r = i < f ? (double)bv[t] - s * ys[i] : 0.0;
         rr += r * r;
         rh[t] = (float)r * sc[t];
+        if (scaled) xv[t] = (float)(s * (double)xv[t]);
       }
       rr = wave_sum_d(rr);
       if (round == 0) rel0 = (float)sqrt(rr / bb);
       rel = (float)sqrt(rr / bb);
       rounds_run = round;
-      if (rr <= tol2) {
+      if (rr <= tol2 && !scaled) {
         accepted = true;
         break;
       }

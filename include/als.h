@@ -93,7 +93,7 @@ typedef struct cumf_options {
   int32_t version;      /* CUMF_OPTIONS_VERSION */
   int32_t precision;    /* CUMF_PREC_* */
   int32_t split;        /* max ratings per wavefront task; longer rows are
-                           split and their partial Grams reduced (0: 2048) */
+                           split and their partial Grams reduced (0: 16384) */
   int32_t timing;       /* 1: time every kernel launch with HIP events */
   int32_t use_graph;    /* 1: cumf_iterate replays one iteration (X sweep +
                            Theta sweep) captured once as a hipGraph; per-kernel
@@ -107,7 +107,7 @@ typedef struct cumf_options {
                            takes it from CUMF_COMPAT_HERMITIAN_BUG=1. */
   int32_t solver;       /* CUMF_SOLVER_* */
   int32_t cg_iters;     /* CG steps per refinement round (0: 6) */
-  int32_t cg_rounds;    /* refinement rounds, at most (0: 3) */
+  int32_t cg_rounds;    /* refinement rounds, at most (0: 6) */
   float cg_tol;         /* accept ||b - A x|| <= cg_tol ||b|| (0: 1e-3) */
   int32_t warm_start;   /* CUMF_WARM_* (CG only) */
   int32_t reserved[5];
@@ -206,7 +206,8 @@ int cumf_csr_to_csc_device(hipStream_t stream, int64_t m, int64_t n, int64_t nnz
  * budget (1 or 2 wavefronts per SIMD) of the split-row finish and of the CG
  * fallback kernels, and per bucket the register budget and whether the
  * bucket runs as a persistent grid of N wavefronts taking tasks from a
- * counter.  Without a table every half-sweep is one launch of all tasks.
+ * counter.  Without a table every half-sweep is one launch of all tasks,
+ * with two wavefronts per SIMD up to f = 124 and one above.
  * ---------------------------------------------------------------------- */
 int cumf_load_tile_table(cumf_handle* h, const char* path);
 int cumf_save_tile_table(cumf_handle* h, const char* path);
@@ -241,9 +242,10 @@ int cumf_debug_compat_system(cumf_handle* h, int side, const int32_t* rows, int6
                              double* out);
 /* Runs one half-sweep of side 0 (X) or 1 (Theta) as cumf_update_x /
  * cumf_update_theta do, recording for each row solved by CUMF_SOLVER_CG:
- * out[4 u .. 4 u + 3] = initial and final relative residual
- * ||b - A x|| / ||b|| (FP64, NaN if CG did not start), refinement rounds
- * run, 1 if the row fell back to Cholesky.  out: host, rows x 4. */
+ * out[4 u .. 4 u + 3] = relative residual ||b - A x|| / ||b|| (FP64, NaN
+ * if CG did not start) of the start (after its rescaling, if any) and of
+ * the result, refinement rounds run, 1 if the row fell back to Cholesky.
+ * out: host, rows x 4. */
 int cumf_debug_cg_trace(cumf_handle* h, int side, float* out);
 /* 1: mallocBest skips device memory and takes its mapped-host fallback
  * (also set by the environment variable CUMF_FORCE_HOST_ALLOC=1). */

@@ -5,8 +5,10 @@ The solver runs FP16 conjugate gradients on the Jacobi-scaled system
 inside FP32 iterative refinement with an FP64 residual, and hands rows
 whose relative residual stays above cg_tol to the Cholesky solve.  Bars:
   * every row: ||b - A x|| / ||b|| <= cg_tol (1 + 1e-3) against the Gram the
-    GPU formed (evaluated here in FP64), and <= 2 cg_tol against the
-    oracle's FP64 system;
+    GPU formed (evaluated here in FP64; 2 cg_tol for rows split across
+    tasks, whose one-pass dump sums in another order than the partial
+    Grams the solver used), and <= 2 cg_tol against the oracle's FP64
+    system;
   * a tolerance FP32 cannot reach sends every row to the Cholesky solve,
     whose factors then match the Cholesky solver's to 1e-6;
   * full training: held-out RMSE within 1e-3 of the FP64 oracle.
@@ -50,14 +52,23 @@ def test_cg_half_sweep_residuals(f, tol):
         D = h.row_gram(0, rows).astype(np.float64)
     assert st["failed_rows"] == 0
     ldf = (f + 3) // 4 * 4
-    worst_gpu = worst_64 = 0.0
+    counts = np.diff(ds.csr_row_ptr)
+    worst_gpu = worst_split = worst_64 = 0.0
     for k, u in enumerate(rows):
         A, b = D[k, :f, :f], D[k, :f, ldf]
         x = X1[u].astype(np.float64)
-        worst_gpu = max(worst_gpu, np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+        res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        if counts[u] <= 256:
+            worst_gpu = max(worst_gpu, res)
+        else:
+            worst_split = max(worst_split, res)
         A64, b64 = O.row_system(ds.csr_row_ptr, ds.csr_col, ds.csr_val, T0, f, lam, u)
         worst_64 = max(worst_64, np.linalg.norm(b64 - A64 @ x) / np.linalg.norm(b64))
+    # unsplit rows: the dumped Gram is the one the solver used, bit for bit
     assert worst_gpu <= tol * (1 + 1e-3), worst_gpu
+    # split rows were solved on a sum of 256-rating partial Grams; the dump
+    # forms the Gram in one pass (another FP32 summation order)
+    assert worst_split <= 2 * tol, worst_split
     assert worst_64 <= 2 * tol, worst_64
     # split rows (heavy) went through the finish kernel's copy of the solver
     Xo, _ = O.half_sweep(ds.csr_row_ptr, ds.csr_col, ds.csr_val, T0, lam)
