@@ -246,13 +246,17 @@ int timed(cumf_handle* h, int kind, F&& launch) {
 int default_variant(int f) { return cumf::sweep_nb(f) >= 9 ? 1 : 2; }
 
 hipError_t task_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
-  return variant == 1 ? cumf::w1::launch_task_kernel(a, s) : cumf::w2::launch_task_kernel(a, s);
+  if (a.solver == CUMF_SOLVER_CG)
+    return variant == 1 ? cumf::w1::cg::launch_task_kernel(a, s) : cumf::w2::cg::launch_task_kernel(a, s);
+  return variant == 1 ? cumf::w1::chol::launch_task_kernel(a, s) : cumf::w2::chol::launch_task_kernel(a, s);
 }
 hipError_t finish_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
-  return variant == 1 ? cumf::w1::launch_finish_kernel(a, s) : cumf::w2::launch_finish_kernel(a, s);
+  if (a.solver == CUMF_SOLVER_CG)
+    return variant == 1 ? cumf::w1::cg::launch_finish_kernel(a, s) : cumf::w2::cg::launch_finish_kernel(a, s);
+  return variant == 1 ? cumf::w1::chol::launch_finish_kernel(a, s) : cumf::w2::chol::launch_finish_kernel(a, s);
 }
 hipError_t fallback_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
-  return variant == 1 ? cumf::w1::launch_fallback_kernel(a, s) : cumf::w2::launch_fallback_kernel(a, s);
+  return variant == 1 ? cumf::w1::chol::launch_fallback_kernel(a, s) : cumf::w2::chol::launch_fallback_kernel(a, s);
 }
 
 // Launch of one degree bucket's tasks with its shape.
@@ -1472,8 +1476,8 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
     CUMF_HIP_TRY(cumf::prepare_compat(f));
     CUMF_HIP_TRY(cumf::launch_compat(c, m, stream));
   } else {
-    CUMF_HIP_TRY(cumf::w2::launch_task_kernel(a, stream));
-    CUMF_HIP_TRY(cumf::w2::launch_finish_kernel(a, stream));
+    CUMF_HIP_TRY(task_launch(a, default_variant(f), stream));
+    CUMF_HIP_TRY(finish_launch(a, default_variant(f), stream));
   }
   if (ldf != f)
     CUMF_HIP_TRY(hipMemcpy2DAsync(d_x, sizeof(float) * f, x, sizeof(float) * ldf, sizeof(float) * f, m,

@@ -736,14 +736,21 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
     // Blocks that lie wholly inside the f x f8 window (all but the last
     // row / column block) store without per-lane predicates; the test is
     // uniform across the wavefront.
+    // The column scales stay in registers; each block row's scales are read
+    // one block ahead, so no store waits on a scale read.
+    float scc[NB];
+#pragma unroll
+    for (int J = 0; J < NB; ++J) scc[J] = ss[16 * J + c];
+    f4 srn = *reinterpret_cast<const f4*>(ss + 4 * g);
 #pragma unroll
     for (int I = 0; I < NB; ++I) {
-      const f4 sr = *reinterpret_cast<const f4*>(ss + 16 * I + 4 * g);
+      const f4 sr = srn;
+      if (I + 1 < NB) srn = *reinterpret_cast<const f4*>(ss + 16 * (I + 1) + 4 * g);
       const bool rows_in = 16 * I + 16 <= f;
 #pragma unroll
       for (int J = I; J < NB; ++J) {
         const int j = 16 * J + c;
-        const float scj = ss[j];
+        const float scj = scc[J];
         const f4 v = acc[tile_index(I, J, NB)];
         h4 tv;
 #pragma unroll

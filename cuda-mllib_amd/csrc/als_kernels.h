@@ -95,10 +95,19 @@ int dump_dim(int f);
 hipError_t launch_fallback_reset(int* count, int n, hipStream_t s);
 // Half-sweep kernels (als_sweep.hip), one set per register budget:
 // w2 = two wavefronts per SIMD, w1 = one.
+// Launchers of the sweep kernels, per register budget (w1, w2) and solver
+// family (chol: Cholesky task/finish kernels and the CG path's fallback;
+// cg: CG task/finish kernels).
 #define CUMF_SWEEP_LAUNCHERS                                                \
+  namespace chol {                                                          \
   hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s);         \
   hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s);       \
-  hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s);
+  hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s);     \
+  }                                                                         \
+  namespace cg {                                                            \
+  hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s);         \
+  hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s);       \
+  }
 namespace w1 { CUMF_SWEEP_LAUNCHERS }
 namespace w2 { CUMF_SWEEP_LAUNCHERS }
 #undef CUMF_SWEEP_LAUNCHERS

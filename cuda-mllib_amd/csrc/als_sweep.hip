@@ -1,12 +1,19 @@
 // als_sweep.hip -- the half-sweep kernels proper: fused Gram + solve per
 // task, the split-row finish and the Cholesky fallback of the CG path.
 //
-// Compiled twice from this one source, once per register budget (the
-// CUMF_TASK_WAVES wavefronts per SIMD the compiler may assume when it
-// allocates registers): 2 (namespace cumf::w2) lets two solves share a SIMD;
-// 1 (cumf::w1) gives a wavefront the whole register file, which the larger
-// tile grids (f > 112) need to keep the Gram out of scratch memory.  The
-// tile table picks one per degree bucket.
+// Compiled four times from this one source:
+//   * per register budget (the CUMF_TASK_WAVES wavefronts per SIMD the
+//     compiler may assume when it allocates registers): 2 (namespace
+//     cumf::w2) lets two solves share a SIMD; 1 (cumf::w1) gives a wavefront
+//     the whole register file, which the larger tile grids (f > 124) need
+//     to keep the Gram out of scratch memory.  The tile table picks one per
+//     degree bucket;
+//   * per solver family (CUMF_SWEEP_SOLVER): 0 (namespace ...::chol) holds
+//     the Cholesky task and finish kernels and the CG path's Cholesky
+//     fallback, 1 (...::cg) the CG task and finish kernels.  The CG object
+//     is built with the scheduler's register-pressure trackers, which keep
+//     its FP64 residual out of scratch memory; the Cholesky kernels run
+//     faster without them.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -22,9 +29,20 @@
 #else
 #error "CUMF_TASK_WAVES must be 1 or 2"
 #endif
+#ifndef CUMF_SWEEP_SOLVER
+#define CUMF_SWEEP_SOLVER 0
+#endif
+#if CUMF_SWEEP_SOLVER == 0
+#define CUMF_SNS chol
+#elif CUMF_SWEEP_SOLVER == 1
+#define CUMF_SNS cg
+#else
+#error "CUMF_SWEEP_SOLVER must be 0 (Cholesky) or 1 (CG)"
+#endif
 
 namespace cumf {
 namespace CUMF_VNS {
+namespace CUMF_SNS {
 
 // One task: Gram of its ratings, then either the partial Gram of a split
 // row goes to scratch, or the row is solved.
@@ -158,35 +176,31 @@ constexpr int kFallbackGrid = 2048;  // work-groups walking the fallback list
 
 template <int NB>
 static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which) {
+  constexpr bool CG = CUMF_SWEEP_SOLVER == 1;
   const bool bf = a.precision != 0;
-  const bool cg = a.solver != 0;
-  const size_t shmem = sizeof(float) * (size_t)cmax(lds_floats_nb(NB, a.f), cg ? cg_lds_floats(NB, a.f) : 0);
+  const size_t shmem = sizeof(float) * (size_t)cmax(lds_floats_nb(NB, a.f), CG ? cg_lds_floats(NB, a.f) : 0);
   if (which == 0) {
     if (a.ntasks > 0) {
       if (a.persistent > 0) {
         const dim3 grid((unsigned)(a.ntasks < a.persistent ? a.ntasks : a.persistent));
-        if (bf && cg) hipLaunchKernelGGL((als_task_kernel<NB, true, true, true>), grid, dim3(64), shmem, s, a);
-        else if (bf) hipLaunchKernelGGL((als_task_kernel<NB, true, false, true>), grid, dim3(64), shmem, s, a);
-        else if (cg) hipLaunchKernelGGL((als_task_kernel<NB, false, true, true>), grid, dim3(64), shmem, s, a);
-        else hipLaunchKernelGGL((als_task_kernel<NB, false, false, true>), grid, dim3(64), shmem, s, a);
+        if (bf) hipLaunchKernelGGL((als_task_kernel<NB, true, CG, true>), grid, dim3(64), shmem, s, a);
+        else hipLaunchKernelGGL((als_task_kernel<NB, false, CG, true>), grid, dim3(64), shmem, s, a);
       } else {
         const dim3 grid((unsigned)a.ntasks);
-        if (bf && cg) hipLaunchKernelGGL((als_task_kernel<NB, true, true, false>), grid, dim3(64), shmem, s, a);
-        else if (bf) hipLaunchKernelGGL((als_task_kernel<NB, true, false, false>), grid, dim3(64), shmem, s, a);
-        else if (cg) hipLaunchKernelGGL((als_task_kernel<NB, false, true, false>), grid, dim3(64), shmem, s, a);
-        else hipLaunchKernelGGL((als_task_kernel<NB, false, false, false>), grid, dim3(64), shmem, s, a);
+        if (bf) hipLaunchKernelGGL((als_task_kernel<NB, true, CG, false>), grid, dim3(64), shmem, s, a);
+        else hipLaunchKernelGGL((als_task_kernel<NB, false, CG, false>), grid, dim3(64), shmem, s, a);
       }
     }
   } else if (which == 1) {
-    if (a.nheavy > 0) {
-      const dim3 grid((unsigned)a.nheavy);
-      if (cg) hipLaunchKernelGGL((als_finish_kernel<NB, true>), grid, dim3(64), shmem, s, a);
-      else hipLaunchKernelGGL((als_finish_kernel<NB, false>), grid, dim3(64), shmem, s, a);
-    }
-  } else if (which == 3) {
+    if (a.nheavy > 0)
+      hipLaunchKernelGGL((als_finish_kernel<NB, CG>), dim3((unsigned)a.nheavy), dim3(64), shmem, s, a);
+  } else if constexpr (!CG) {
+    // which == 3: the CG path's Cholesky fallback (lives with the Cholesky kernels)
     const size_t shc = sizeof(float) * (size_t)lds_floats_nb(NB, a.f);
     if (bf) hipLaunchKernelGGL((als_fallback_kernel<NB, true>), dim3(kFallbackGrid), dim3(64), shc, s, a);
     else hipLaunchKernelGGL((als_fallback_kernel<NB, false>), dim3(kFallbackGrid), dim3(64), shc, s, a);
+  } else {
+    return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
@@ -207,8 +221,11 @@ static hipError_t dispatch(const SweepArgs& a, hipStream_t s, int which) {
 }
 
 hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 0); }
-hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 3); }
 hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 1); }
+#if CUMF_SWEEP_SOLVER == 0
+hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 3); }
+#endif
 
+}  // namespace CUMF_SNS
 }  // namespace CUMF_VNS
 }  // namespace cumf
