@@ -529,6 +529,26 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
+// Every lane gets its value plus that of lane l ^ 16 (l ^ 32): the
+// permlane swaps (two VALU moves, no LDS) exchange whole 16-lane (32-lane)
+// rows; both lanes of a pair add the same two values.
+__device__ __forceinline__ double swap_sum16(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane16_swap((int)(b & 0xffffffff), (int)(b & 0xffffffff), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((int)(b >> 32), (int)(b >> 32), false, false);
+  const double a = __longlong_as_double(((int64_t)hi[0] << 32) | (uint32_t)lo[0]);
+  const double o = __longlong_as_double(((int64_t)hi[1] << 32) | (uint32_t)lo[1]);
+  return a + o;
+}
+__device__ __forceinline__ double swap_sum32(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((int)(b & 0xffffffff), (int)(b & 0xffffffff), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((int)(b >> 32), (int)(b >> 32), false, false);
+  const double a = __longlong_as_double(((int64_t)hi[0] << 32) | (uint32_t)lo[0]);
+  const double o = __longlong_as_double(((int64_t)hi[1] << 32) | (uint32_t)lo[1]);
+  return a + o;
+}
+
 // A register value the compiler must treat as changed at this point: keeps
 // loop-invariant FP64 conversions of the Gram from being hoisted out of the
 // refinement loop (which would hold 2x the Gram's registers live).
@@ -541,9 +561,13 @@ __device__ __forceinline__ f4 opaque(f4 v) {
 // diagonal tiles are full), xs zero beyond f.  Row direction: lane (g, c)
 // of tile (I, J) adds A[16I+4g+r][16J+c] x[16J+c] to row 16I+4g+r; the 16
 // lanes of a group are reduced with a reduce-scatter that leaves row
-// 16I+4g+(c>>2) in lanes c with equal c>>2.  Column direction (strictly
-// upper tiles, the lower triangle by symmetry): row 16J+c collects
-// A[16I+4g+r][16J+c] x[16I+4g+r] over r, I < J and the four groups.
+// 16I+4g+(c>>2) in lanes c with equal c>>2, all in DPP moves: row_ror:8
+// pairs lane c with c ^ 8 (rows {0,1} stay in the low half, {2,3} in the
+// high half), row_half_mirror pairs each lane with one lane of the other
+// quad of its half (each quad keeps one row), and two quad_perm steps sum
+// the quad.  Column direction (strictly upper tiles, the lower triangle by
+// symmetry): row 16J+c collects A[16I+4g+r][16J+c] x[16I+4g+r] over r,
+// I < J and the four groups (permlane swaps across the 16-lane rows).
 template <int NB>
 __device__ __forceinline__ void gram_matvec_f64(const f4 (&acc)[num_tiles(NB)], const float* __restrict__ xs,
                                                 double* __restrict__ ys, int lane) {
@@ -565,13 +589,13 @@ __device__ __forceinline__ void gram_matvec_f64(const f4 (&acc)[num_tiles(NB)], 
     }
     double k0 = h8 ? y2 : y0, k1 = h8 ? y3 : y1;
     const double s0 = h8 ? y0 : y2, s1 = h8 ? y1 : y3;
-    k0 += __shfl_xor(s0, 8, 64);
-    k1 += __shfl_xor(s1, 8, 64);
+    k0 += dpp_d<0x128>(s0);  // row_ror:8 (lane c ^ 8)
+    k1 += dpp_d<0x128>(s1);
     double k = h4b ? k1 : k0;
     const double s = h4b ? k0 : k1;
-    k += __shfl_xor(s, 4, 64);
-    k += __shfl_xor(k, 2, 64);
-    k += __shfl_xor(k, 1, 64);
+    k += dpp_d<0x141>(s);    // row_half_mirror (lane 7 - c within the half)
+    k += dpp_d<0x4E>(k);     // quad_perm [2,3,0,1]
+    k += dpp_d<0xB1>(k);     // quad_perm [1,0,3,2]
     if ((c & 3) == 0) ys[16 * I + 4 * g + (c >> 2)] = k;
   }
   lds_order();
@@ -585,8 +609,7 @@ __device__ __forceinline__ void gram_matvec_f64(const f4 (&acc)[num_tiles(NB)], 
 #pragma unroll
       for (int r = 0; r < 4; ++r) yc = fma((double)v[r], (double)xr[r], yc);
     }
-    yc += __shfl_xor(yc, 16, 64);
-    yc += __shfl_xor(yc, 32, 64);
+    yc = swap_sum32(swap_sum16(yc));
     if (g == 0) ys[16 * J + c] += yc;
   }
   lds_order();
