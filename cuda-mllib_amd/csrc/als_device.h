@@ -615,6 +615,64 @@ __device__ __forceinline__ void gram_matvec_f64(const f4 (&acc)[num_tiles(NB)], 
   lds_order();
 }
 
+// ysf[0, 16NB) = A xs with FP32 products and sums: the same tile walk and
+// reductions as gram_matvec_f64, for the start's residual, which only
+// drives the first CG round (no acceptance test reads it).
+__device__ __forceinline__ float swap_sum16f(float v) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(v), __float_as_int(v), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+__device__ __forceinline__ float swap_sum32f(float v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(v), __float_as_int(v), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+template <int NB>
+__device__ __forceinline__ void gram_matvec_f32(const f4 (&acc)[num_tiles(NB)], const float* __restrict__ xs,
+                                                float* __restrict__ ysf, int lane) {
+  const int g = lane >> 4, c = lane & 15;
+  float xc[NB];
+#pragma unroll
+  for (int J = 0; J < NB; ++J) xc[J] = xs[16 * J + c];
+  const bool h8 = (c & 8) != 0, h4b = (c & 4) != 0;
+#pragma unroll
+  for (int I = 0; I < NB; ++I) {
+    float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f, y3 = 0.0f;
+#pragma unroll
+    for (int J = I; J < NB; ++J) {
+      const f4 v = acc[tile_index(I, J, NB)];
+      y0 = fmaf(v[0], xc[J], y0);
+      y1 = fmaf(v[1], xc[J], y1);
+      y2 = fmaf(v[2], xc[J], y2);
+      y3 = fmaf(v[3], xc[J], y3);
+    }
+    float k0 = h8 ? y2 : y0, k1 = h8 ? y3 : y1;
+    const float s0 = h8 ? y0 : y2, s1 = h8 ? y1 : y3;
+    k0 += dpp_f<0x128>(s0);
+    k1 += dpp_f<0x128>(s1);
+    float k = h4b ? k1 : k0;
+    const float s = h4b ? k0 : k1;
+    k += dpp_f<0x141>(s);
+    k += dpp_f<0x4E>(k);
+    k += dpp_f<0xB1>(k);
+    if ((c & 3) == 0) ysf[16 * I + 4 * g + (c >> 2)] = k;
+  }
+  lds_order();
+#pragma unroll
+  for (int J = 1; J < NB; ++J) {
+    float yc = 0.0f;
+#pragma unroll
+    for (int I = 0; I < J; ++I) {
+      const f4 xr = *reinterpret_cast<const f4*>(xs + 16 * I + 4 * g);
+      const f4 v = acc[tile_index(I, J, NB)];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) yc = fmaf(v[r], xr[r], yc);
+    }
+    yc = swap_sum32f(swap_sum16f(yc));
+    if (g == 0) ysf[16 * J + c] += yc;
+  }
+  lds_order();
+}
+
 // q = A^ p for the rows this lane owns (FP16 operands, FP32 accumulation).
 // Rows past f are computed on row f-1 (same-address reads broadcast) and
 // zeroed.
@@ -819,15 +877,19 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
       }
       lds_order();
     }
+    float* ysf = reinterpret_cast<float*>(ys);
     for (int round = 0;; ++round) {
-      // FP64 residual of the current x (xs holds it).
-      gram_matvec_f64<NB>(acc, xs, ys, lane);
+      // Residual of the current x (xs holds it): the start's in FP32 (it
+      // only drives the first CG round), every later one, which decides
+      // acceptance, in FP64.
+      if (round == 0) gram_matvec_f32<NB>(acc, xs, ysf, lane);
+      else gram_matvec_f64<NB>(acc, xs, ys, lane);
+      auto yat = [&](int i) -> double { return round == 0 ? (double)ysf[i] : ys[i]; };
       // A start far from the solution (the factor of the previous sweep
       // after the other side changed a lot) is first replaced by its
       // energy-optimal multiple s x0, s = <x0, b> / <x0, A x0>, whose
       // A-norm error is never above that of x0 or of zero.  The residual
-      // b - s A x0 then comes from the same product; acceptance waits for
-      // the next round's residual of the rounded s x0.  Starts with
+      // b - s A x0 then comes from the same product.  Starts with
       // |s - 1| <= 1/8 are kept as they are.
       double s = 1.0;
       if (round == 0 && cp.warm != 1) {
@@ -837,7 +899,7 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
           const int i = lane + 64 * t;
           if (i < f) {
             xbd += (double)xv[t] * (double)bv[t];
-            xad += (double)xv[t] * ys[i];
+            xad += (double)xv[t] * yat(i);
           }
         }
         xbd = wave_sum_d(xbd);
@@ -851,7 +913,7 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
         const int i = lane + 64 * t;
-        const double r = i < f ? (double)bv[t] - s * ys[i] : 0.0;
+        const double r = i < f ? (double)bv[t] - s * yat(i) : 0.0;
         rr += r * r;
         rh[t] = (float)r * sc[t];
         if (scaled) xv[t] = (float)(s * (double)xv[t]);
@@ -860,7 +922,7 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
       if (round == 0) rel0 = (float)sqrt(rr / bb);
       rel = (float)sqrt(rr / bb);
       rounds_run = round;
-      if (rr <= tol2 && !scaled) {
+      if (rr <= tol2 && round > 0) {
         accepted = true;
         break;
       }
