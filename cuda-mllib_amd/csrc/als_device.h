@@ -755,21 +755,28 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
   // Diagonal and right-hand side out of the register tiles.  Lane (g, c)
   // of diagonal tile I holds a_ii for i = 16I + c when c >> 2 == g (its
   // register c & 3); the right-hand side is the column bcol (bcol2) of the
-  // block-column bcol >> 4, held by the lanes with c == bcol & 15.  One
-  // store per lane and block (rows past f are written too and never read).
+  // block-column bcol >> 4, held by the lanes with c == bcol & 15.  Every
+  // lane stores (rows past f are written too and never read); lanes that
+  // hold nothing wanted write into a sink at the start of the matrix area
+  // (16 bytes per lane where the area holds 1 KB, shared by lanes for small
+  // f), which the FP16 matrix overwrites later.  No store is predicated, so
+  // the set-up has no exec-mask branches.
   lds_order();
   const int jb = bcol >> 4, cb = bcol & 15;
   const int jb2 = bcol2 >= 0 ? bcol2 >> 4 : -1, cb2 = bcol2 & 15;
+  const int span = min(256, (f * sa) >> 1);  // floats of the matrix area, capped (>= 4)
+  float* sink = lds + ((4 * lane) & ((1 << (31 - __clz(span))) - 1));
 #pragma unroll
   for (int I = 0; I < NB; ++I) {
     const f4 dv = acc[tile_index(I, I, NB)];
-    const int rr = c & 3;
-    const float dval = rr == 0 ? dv[0] : (rr == 1 ? dv[1] : (rr == 2 ? dv[2] : dv[3]));
-    if ((c >> 2) == g) ds[16 * I + c] = dval;
+    const bool odd = (c & 1) != 0;
+    const float dlo = odd ? dv[1] : dv[0], dhi = odd ? dv[3] : dv[2];
+    const float dval = (c & 2) ? dhi : dlo;  // register c & 3, as selects
+    *((c >> 2) == g ? ds + 16 * I + c : sink) = dval;
 #pragma unroll
     for (int J = I; J < NB; ++J) {
-      if (J == jb && c == cb) *reinterpret_cast<f4*>(bs + 16 * I + 4 * g) = acc[tile_index(I, J, NB)];
-      if (J == jb2 && c == cb2) *reinterpret_cast<f4*>(bs + D + 16 * I + 4 * g) = acc[tile_index(I, J, NB)];
+      if (J == jb) *reinterpret_cast<f4*>(c == cb ? bs + 16 * I + 4 * g : sink) = acc[tile_index(I, J, NB)];
+      if (J == jb2) *reinterpret_cast<f4*>(c == cb2 ? bs + D + 16 * I + 4 * g : sink) = acc[tile_index(I, J, NB)];
     }
   }
   lds_order();
@@ -813,10 +820,13 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
   int rounds_run = 0;
   if (!any_bad) {
     // A^ in FP16: tile (I, J) writes rows 16I+4g+r at column 16J+c and, for
-    // I < J, row 16J+c at columns 16I+4g..+3 (one 8-byte store).
-    // Blocks that lie wholly inside the f x f8 window (all but the last
-    // row / column block) store without per-lane predicates; the test is
-    // uniform across the wavefront.
+    // I < J, row 16J+c at columns 16I+4g..+3 (one 8-byte store).  Entries
+    // outside the f x f8 window go to the row after the matrix (the CG
+    // direction vector, sa >= 8 halves, rewritten before every read), so
+    // no store is predicated.  NB = sweep_nb(f) gives f >= 16 NB - 22, f8 >= 16 NB - 16:
+    // row blocks up to NB-3 and column blocks up to NB-2 are always inside
+    // (compile-time), and a transposed store's columns 16I+4g+3 <=
+    // 16 NB - 17 always are.
     // The column scales stay in registers; each block row's scales are read
     // one block ahead, so no store waits on a scale read.
     float scc[NB];
@@ -827,7 +837,6 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
     for (int I = 0; I < NB; ++I) {
       const f4 sr = srn;
       if (I + 1 < NB) srn = *reinterpret_cast<const f4*>(ss + 16 * (I + 1) + 4 * g);
-      const bool rows_in = 16 * I + 16 <= f;
 #pragma unroll
       for (int J = I; J < NB; ++J) {
         const int j = 16 * J + c;
@@ -837,18 +846,15 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
 #pragma unroll
         for (int r = 0; r < 4; ++r) tv[r] = (_Float16)(v[r] * sr[r] * scj);
         _Float16* drow = Ah + (16 * I + 4 * g) * sa + j;
-        if (rows_in && 16 * J + 16 <= f8) {
+        const bool col_ok = J <= NB - 2 || j < f8;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) drow[r * sa] = tv[r];
-        } else if (j < f8) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (16 * I + 4 * g + r < f) drow[r * sa] = tv[r];
+        for (int r = 0; r < 4; ++r) {
+          const bool ok = col_ok && (I <= NB - 3 || 16 * I + 4 * g + r < f);
+          *(ok ? drow + r * sa : p16 + (c & 7)) = tv[r];
         }
         if (I < J) {
-          h4* dt = reinterpret_cast<h4*>(Ah + j * sa + 16 * I + 4 * g);
-          if (16 * J + 16 <= f && 16 * I + 16 <= f8) *dt = tv;
-          else if (j < f && 16 * I + 4 * g < f8) *dt = tv;
+          const bool ok = J <= NB - 3 || j < f;
+          *reinterpret_cast<h4*>(ok ? Ah + j * sa + 16 * I + 4 * g : p16 + 16 * I + 4 * g) = tv;
         }
       }
     }
