@@ -689,7 +689,7 @@ __device__ __forceinline__ void cg_matvec(const _Float16* __restrict__ Ah, const
     q[t] = 0.0f;
     q2[t] = 0.0f;
   }
-#pragma unroll 2
+#pragma unroll 4
   for (int k = 0; k < nch; ++k) {
     const h8 pk = *reinterpret_cast<const h8*>(p16 + 8 * k);
 #pragma unroll
@@ -962,7 +962,7 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
         for (int t = 0; t < RPL; ++t) pq = fmaf(pv[t], q[t], pq);
         pq = wave_sum(pq);
         if (!(pq > 0.0f) || !(rho > 0.0f)) break;
-        const float alpha = rho / pq;
+        const float alpha = rho * __builtin_amdgcn_rcpf(pq);  // step lengths need no IEEE division
         float rn = 0.0f;
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
@@ -971,7 +971,7 @@ __device__ __forceinline__ bool cg_solve(f4 (&acc)[num_tiles(NB)], int f, int bc
           rn = fmaf(rv[t], rv[t], rn);
         }
         rn = wave_sum(rn);
-        const float beta = rn / rho;
+        const float beta = rn * __builtin_amdgcn_rcpf(rho);
         rho = rn;
 #pragma unroll
         for (int t = 0; t < RPL; ++t) pv[t] = fmaf(beta, pv[t], rv[t]);
