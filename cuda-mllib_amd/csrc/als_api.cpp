@@ -275,6 +275,17 @@ hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::Sw
 // counters are cleared.
 int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs& a) {
   if (!h->tiled) return timed(h, Timer::TASK, [&] { return task_launch(a, default_variant(h->f), h->stream); });
+  // A side whose non-empty buckets share one shape and no persistent grid is
+  // one launch (launches in sequence drain the GPU between buckets).
+  int v0 = 0;
+  bool uniform = true;
+  for (int b = 0; b < kBuckets; ++b) {
+    if (p.bucket_t1[b] <= p.bucket_t0[b]) continue;
+    const BucketCfg& c = h->tiles[side][b];
+    if (v0 == 0) v0 = c.variant;
+    uniform = uniform && c.persistent == 0 && c.variant == v0;
+  }
+  if (uniform && v0 != 0) return timed(h, Timer::TASK, [&] { return task_launch(a, v0, h->stream); });
   bool persistent = false;
   for (int b = 0; b < kBuckets; ++b) persistent = persistent || h->tiles[side][b].persistent > 0;
   if (persistent)
@@ -1045,6 +1056,25 @@ int autotune_side(cumf_handle* h, int side, int reps) {
       h->fallback_variant[side] = t1 < t2 ? 1 : 2;
       CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
     }
+  }
+  // 4. Buckets launched one after another drain the GPU between launches:
+  //    keep the per-bucket shapes only if a whole half-sweep with them beats
+  //    one launch of every task with the default shape.
+  {
+    BucketCfg tuned[kBuckets];
+    std::copy(h->tiles[side], h->tiles[side] + kBuckets, tuned);
+    const bool was_tiled = h->tiled;
+    h->tiled = true;
+    auto sweep = [&] {
+      SideBufs sb = side_bufs(h, side);
+      return run_half_sweep(h, side, *sb.p, *sb.rowp, *sb.col, *sb.val, sb.theta, sb.x, *sb.theta_b, *sb.x_b);
+    };
+    double t_buckets = 0, t_single = 0;
+    CUMF_TRY(time_min_ms(h, reps, sweep, &t_buckets));
+    for (int k = 0; k < kBuckets; ++k) h->tiles[side][k] = BucketCfg{default_variant(h->f), 0};
+    CUMF_TRY(time_min_ms(h, reps, sweep, &t_single));
+    if (t_buckets < t_single) std::copy(tuned, tuned + kBuckets, h->tiles[side]);
+    h->tiled = was_tiled;
   }
   return CUMF_OK;
 }
