@@ -76,11 +76,13 @@ void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, f
 
 #define CUMF_SOLVER_CHOLESKY 0  /* FP32 Cholesky of every row's system */
 #define CUMF_SOLVER_CG 1        /* FP16 conjugate gradients on the Jacobi-scaled
-                                   system inside FP32 iterative refinement, the
-                                   residual b - A x formed in FP64, started from
-                                   the row's current factor; rows whose relative
-                                   residual stays above cg_tol fall back to the
-                                   Cholesky solve */
+                                   system inside FP32 iterative refinement, every
+                                   residual b - A x that can accept a row formed
+                                   in FP64 (the start's, which only drives the
+                                   first round, in FP32), started from the row's
+                                   current factor; rows whose relative residual
+                                   stays above cg_tol fall back to the Cholesky
+                                   solve */
 
 #define CUMF_WARM_PREV 0        /* CG starts from the row's current factor */
 #define CUMF_WARM_ZERO 1        /* CG starts from zero */
@@ -95,9 +97,9 @@ typedef struct cumf_options {
   int32_t split;        /* max ratings per wavefront task; longer rows are
                            split and their partial Grams reduced (0: 16384) */
   int32_t timing;       /* 1: time every kernel launch with HIP events */
-  int32_t use_graph;    /* 1: cumf_iterate replays one iteration (X sweep +
-                           Theta sweep) captured once as a hipGraph; per-kernel
-                           times are then replaced by per-iteration times */
+  int32_t use_graph;    /* 1: each half-sweep (X, Theta) is replayed from a
+                           hipGraph captured at its first run; per-kernel
+                           times are then replaced by per-replay times */
   int32_t compat_hermitian_bug; /* 1: reproduce the reference's arithmetic and
                            its Gram write slip (als/src/cuda/als.cu:275, 718):
                            sequential FP32 Gram, element (10I+4, 10J) :=
