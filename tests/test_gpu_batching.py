@@ -120,3 +120,16 @@ def test_bad_tile_table_rejected(tmp_path):
         p.write_text("bucket 0 9 0 0 variant 2 persistent 0\n")
         with pytest.raises(G.CumfError):
             h.load_tile_table(str(p))
+
+
+def test_uniform_table_is_one_launch(tmp_path):
+    """A side whose buckets share one shape and no persistent grid runs as one
+    launch of all its tasks, with the factors of the table-free path."""
+    ds = _data.netflix_like_small()
+    f, lam = 32, 0.058
+    T0 = cumf.init_factors(ds.n, f, 44)
+    Xr, Tr, sr = _run(ds, f, lam, T0, iters=2, split=512, solver=G.CUMF_SOLVER_CG)
+    tab = _table(tmp_path, 512, lambda s, b: 2, lambda s, b: 0)
+    Xu, Tu, su = _run(ds, f, lam, T0, iters=2, table=tab, solver=G.CUMF_SOLVER_CG)
+    assert _rel(Xu, Xr) <= 1e-6 and _rel(Tu, Tr) <= 1e-6
+    assert sr["task_launches"] == 4 and su["task_launches"] == 4  # one per half-sweep
