@@ -280,7 +280,7 @@ def main():
         path = ("fp32" if prec == G.CUMF_PREC_FP32 else "bf16") + " gram (MFMA) + fp32 cholesky"
     else:
         path = (("fp32" if prec == G.CUMF_PREC_FP32 else "bf16") + " gram (MFMA, fp32 accumulate) + "
-                f"fp16 cg ({opt.cg_iters} steps x {opt.cg_rounds} fp32 refinement rounds, fp64 residual, "
+                f"fp16 cg ({opt.cg_iters} steps x up to {opt.cg_rounds} fp32 refinement rounds, fp64 acceptance residual, "
                 f"tol {opt.cg_tol:g}, cholesky fallback, start {args.warm})")
     rec = {
         "metric": METRIC,

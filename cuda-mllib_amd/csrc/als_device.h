@@ -20,7 +20,7 @@
 //     z = U^{-T} b in the augmented column; U and z go to LDS for the back
 //     substitution.
 //   * Or the mixed-precision solve (cg_solve): FP16 CG on the Jacobi-scaled
-//     matrix in LDS inside FP32 refinement with an FP64 residual.
+//     matrix in LDS inside FP32 refinement with FP64 acceptance residuals.
 //   * Rows with more than `split` ratings are cut into chunks whose partial
 //     Grams go to a scratch buffer in register order; a second kernel sums
 //     them in chunk order (deterministic) and runs the same solve.
@@ -460,13 +460,15 @@ __device__ __forceinline__ void add_ridge(f4 (&acc)[num_tiles(NB)], int f, float
 // Mixed-precision solve: conjugate gradients on the Jacobi-scaled matrix
 // A^ = D^-1/2 A D^-1/2 held in LDS as FP16 (unit diagonal, |entries| <= 1,
 // so FP16 holds it without range trouble), inside FP32 iterative
-// refinement whose residual r = b - A x is formed in FP64 from the FP32
-// Gram still in registers.  Each refinement round solves A^ d = D^-1/2 r /
-// mu (mu = max |D^-1/2 r|, so the FP16 search directions stay O(1)) with a
-// fixed number of CG iterations from d = 0 and adds mu D^-1/2 d to x.  The
-// start is the row's current factor (the previous sweep's value).  A row
-// whose relative residual ||b - A x|| / ||b|| has not reached the tolerance
-// after the last round is solved again by the Cholesky path.
+// refinement whose residual r = b - A x is formed from the FP32 Gram still
+// in registers: in FP64 whenever it can accept the row, in FP32 for the
+// start (it only drives the first round).  Each refinement round solves
+// A^ d = D^-1/2 r / mu (mu = max |D^-1/2 r|, so the FP16 search directions
+// stay O(1)) with a fixed number of CG iterations from d = 0 and adds
+// mu D^-1/2 d to x.  The start is the row's current factor (the previous
+// sweep's value), rescaled when far off.  A row whose relative residual
+// ||b - A x|| / ||b|| has not reached the tolerance after the last round is
+// solved again by the Cholesky path.
 //
 // LDS (after the Gram phase): A^ (f rows, stride cg_stride(f) halves), the
 // FP16 direction p, x (16NB floats) and y = A x (16NB doubles).  Vectors
