@@ -34,7 +34,10 @@ using cumf::DevBuf;
 
 namespace {
 
-constexpr int kMaxF = 140;
+constexpr int kMaxF = 256;
+// Largest f of the fused register-tile kernels (9 column blocks); above it
+// the half-sweep runs the work-group-per-row FP32 Cholesky kernel.
+constexpr int kMaxFusedF = 140;
 constexpr int kDefaultSplit = 16384;
 constexpr int kDefaultCgIters = 6;
 constexpr int kDefaultCgRounds = 6;
@@ -382,6 +385,11 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
   }
   cumf::SweepArgs a{};
   CUMF_TRY(fill_args(h, p, rowp, col, val, theta, x, theta_b, x_b, trace, a));
+  if (h->f > kMaxFusedF) {
+    CUMF_TRY(timed(h, Timer::TASK, [&] { return cumf::launch_large_kernel(a, p.rows, h->stream); }));
+    h->gram_flops += p.nnz * (int64_t)h->f * (h->f + 1);
+    return CUMF_OK;
+  }
   const bool cg = h->opt.solver == CUMF_SOLVER_CG;
   a.warm = h->opt.warm_start;
   if (side == 1 && a.warm >= CUMF_WARM_MLP) a.warm = CUMF_WARM_PREV;  // the network predicts X rows
@@ -483,6 +491,12 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
     cumf::set_last_error("warm_start must be a CUMF_WARM_* value; only CUMF_SOLVER_CG has a start");
     return CUMF_ERR_INVALID;
   }
+  if (f > kMaxFusedF && (o.precision != CUMF_PREC_FP32 || o.solver != CUMF_SOLVER_CHOLESKY ||
+                         o.compat_hermitian_bug)) {
+    cumf::set_last_error("f > " + std::to_string(kMaxFusedF) +
+                         " runs the FP32 Gram + FP32 Cholesky path only");
+    return CUMF_ERR_UNSUPPORTED;
+  }
   if (o.cg_iters > 64 || o.cg_rounds > 16) {
     cumf::set_last_error("cg_iters must be <= 64 and cg_rounds <= 16");
     return CUMF_ERR_INVALID;
@@ -536,6 +550,11 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
   if (rc == CUMF_OK) rc = h->zero16.alloc(64);
   if (rc == CUMF_OK && hipMemset(h->zero16.p, 0, 64) != hipSuccess) rc = CUMF_ERR_HIP;
   if (rc == CUMF_OK && hipMemset(h->fail.p, 0, 2 * sizeof(int)) != hipSuccess) rc = CUMF_ERR_HIP;
+  if (rc == CUMF_OK && f > kMaxFusedF && cumf::prepare_large_kernel(f) != hipSuccess) {
+    (void)hipGetLastError();
+    cumf::set_last_error("large-f kernel: cannot raise its LDS limit");
+    rc = CUMF_ERR_HIP;
+  }
   if (rc != CUMF_OK) {
     (void)hipStreamDestroy(h->stream);
     delete h;
@@ -870,7 +889,7 @@ int cumf_reset_stats(cumf_handle* h) {
 
 hipStream_t cumf_get_stream(cumf_handle* h) { return h ? h->stream : nullptr; }
 
-int cumf_debug_gram_dim(int f) { return (f >= 1 && f <= kMaxF) ? cumf::dump_dim(f) : -1; }
+int cumf_debug_gram_dim(int f) { return (f >= 1 && f <= kMaxFusedF) ? cumf::dump_dim(f) : -1; }
 
 int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t nrows, float* out) {
   cumf::clear_last_error();
@@ -1168,6 +1187,11 @@ int cumf_autotune(cumf_handle* h, int reps, const char* path) {
     cumf::set_last_error("cumf_autotune: no ratings yet");
     return CUMF_ERR_STATE;
   }
+  if (h->f > kMaxFusedF) {
+    cumf::set_last_error("cumf_autotune: f > " + std::to_string(kMaxFusedF) +
+                         " runs the work-group-per-row kernel, which has a single launch shape");
+    return CUMF_ERR_UNSUPPORTED;
+  }
   if (h->opt.compat_hermitian_bug) {
     cumf::set_last_error("cumf_autotune: the compat path has a single launch shape");
     return CUMF_ERR_INVALID;
@@ -1453,6 +1477,10 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
     cumf::set_last_error("compat_hermitian_bug reproduces the reference, which needs f % 10 == 0");
     return CUMF_ERR_INVALID;
   }
+  if (o.compat_hermitian_bug && f > kMaxFusedF) {
+    cumf::set_last_error("compat_hermitian_bug is built for f <= " + std::to_string(kMaxFusedF));
+    return CUMF_ERR_UNSUPPORTED;
+  }
   std::vector<int32_t> rp(m + 1);
   CUMF_HIP_TRY(hipMemcpyAsync(rp.data(), d_row_ptr, sizeof(int32_t) * (m + 1), hipMemcpyDeviceToHost, stream));
   CUMF_HIP_TRY(hipStreamSynchronize(stream));
@@ -1505,6 +1533,9 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
     c.lambda = lambda;
     CUMF_HIP_TRY(cumf::prepare_compat(f));
     CUMF_HIP_TRY(cumf::launch_compat(c, m, stream));
+  } else if (f > kMaxFusedF) {
+    CUMF_HIP_TRY(cumf::prepare_large_kernel(f));
+    CUMF_HIP_TRY(cumf::launch_large_kernel(a, m, stream));
   } else {
     CUMF_HIP_TRY(task_launch(a, default_variant(f), stream));
     CUMF_HIP_TRY(finish_launch(a, default_variant(f), stream));

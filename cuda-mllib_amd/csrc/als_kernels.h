@@ -111,6 +111,13 @@ hipError_t launch_fallback_reset(int* count, int n, hipStream_t s);
 namespace w1 { CUMF_SWEEP_LAUNCHERS }
 namespace w2 { CUMF_SWEEP_LAUNCHERS }
 #undef CUMF_SWEEP_LAUNCHERS
+// Half-sweep for 140 < f <= 256 (FP32 Gram + FP32 Cholesky, one 256-thread
+// work-group per row over row_ptr[0 .. rows]); large_lds_bytes(f) of LDS.
+// prepare_large_kernel(f) raises the kernel's dynamic LDS limit to
+// large_lds_bytes(f); call it outside any stream capture before launching.
+hipError_t launch_large_kernel(const SweepArgs& a, int64_t rows, hipStream_t s);
+hipError_t prepare_large_kernel(int f);
+size_t large_lds_bytes(int f);
 hipError_t launch_dump_kernel(const SweepArgs& a, const int32_t* rows, int64_t nrows, float* out,
                               hipStream_t s);
 hipError_t launch_rmse(const int32_t* u, const int32_t* v, const float* r, int64_t cnt, const float* X,

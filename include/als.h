@@ -47,7 +47,8 @@ typedef struct ihipStream_t* hipStream_t;
  *   - X_BATCH (>= 1) bounded the reference's temporary memory; the fused
  *     kernels here need no per-row f x f buffer, so it is only validated.
  * Differences (documented improvements):
- *   - any 1 <= f <= 140 (the reference needs f % 10 == 0, als/README.md:63);
+ *   - any 1 <= f <= 256 (the reference needs f % 10 == 0, als/README.md:63);
+ *     f > 140 runs a work-group-per-row FP32 Gram + Cholesky kernel;
  *   - rows with no ratings get x_u = 0 (the reference divides by zero);
  *   - errors never exit(): the call returns early and cumf_last_error()
  *     describes the failure (the CSR arrays are still released).
@@ -140,7 +141,9 @@ typedef struct cumf_stats {
 typedef struct cumf_handle cumf_handle;
 
 int cumf_default_options(cumf_options* opt);
-/* device < 0: pick with get_gpu().  f in [1, 140], lambda >= 0. */
+/* device < 0: pick with get_gpu().  f in [1, 256], lambda >= 0.  For f > 140
+ * only CUMF_PREC_FP32 with CUMF_SOLVER_CHOLESKY is built (no compat mode, no
+ * autotuner); other options give CUMF_ERR_UNSUPPORTED. */
 int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_handle** out);
 int cumf_destroy(cumf_handle* h);
 /* Host arrays; copied to the device.  The CSC (= CSR of R^T) is optional:
