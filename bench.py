@@ -279,9 +279,12 @@ def main():
     if solver == G.CUMF_SOLVER_CHOLESKY:
         path = ("fp32" if prec == G.CUMF_PREC_FP32 else "bf16") + " gram (MFMA) + fp32 cholesky"
     else:
+        dflt = G.default_options()  # 0 in an option means the library default
+        iters, rounds = opt.cg_iters or dflt.cg_iters, opt.cg_rounds or dflt.cg_rounds
+        tol = opt.cg_tol if opt.cg_tol > 0 else dflt.cg_tol
         path = (("fp32" if prec == G.CUMF_PREC_FP32 else "bf16") + " gram (MFMA, fp32 accumulate) + "
-                f"fp16 cg ({opt.cg_iters} steps x up to {opt.cg_rounds} fp32 refinement rounds, fp64 acceptance residual, "
-                f"tol {opt.cg_tol:g}, cholesky fallback, start {args.warm})")
+                f"fp16 cg ({iters} steps x up to {rounds} fp32 refinement rounds, fp64 acceptance residual, "
+                f"tol {tol:g}, cholesky fallback, start {args.warm})")
     rec = {
         "metric": METRIC,
         "value": s_per_iter,
