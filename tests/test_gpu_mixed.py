@@ -38,7 +38,8 @@ def _warm_factors(ds, f, lam, iters, seed):
         return h.get_factors()
 
 
-@pytest.mark.parametrize("f,tol", [(10, 1e-4), (32, 1e-4), (100, 1e-4), (100, 1e-6), (128, 1e-4)])
+@pytest.mark.parametrize("f,tol", [(10, 1e-4), (20, 1e-4), (32, 1e-4), (100, 1e-4), (100, 1e-6), (120, 1e-4),
+                                   (128, 1e-4)])
 def test_cg_half_sweep_residuals(f, tol):
     ds = _data.netflix_like_small()
     lam = 0.058
