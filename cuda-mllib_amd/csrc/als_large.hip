@@ -11,10 +11,10 @@
 //     accumulates them in registers with FP32 fused multiply-adds.
 //   * The blocks are written to LDS as the packed upper triangle of the
 //     (f+1) x (f+1) augmented matrix (<= 33,153 floats, 130 KB at f = 256),
-//     the ridge lambda n_u is added to the diagonal, and a right-looking
-//     Cholesky runs in place over a 16 x 16 thread grid, one barrier per
-//     pivot (row k is scaled at the start of step k+1, when nobody reads it
-//     any more); the augmented column becomes z = U^{-T} b.
+//     the ridge lambda n_u is added to the diagonal, and a blocked
+//     right-looking Cholesky runs in place (panels of 8 pivots, then a
+//     rank-8 update of the trailing rows in 4x4 register tiles); the
+//     augmented column becomes z = U^{-T} b.
 //   * Back substitution U x = z in wavefront 0, z in registers.
 // Rows without ratings get x = 0; a non-positive pivot counts a failed row
 // and gives x = 0.
@@ -129,32 +129,86 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
   const float ridge = a.lambda * (float)cnt;
   for (int i = t; i < f; i += kLT) P[row_base(i, F1) + i] += ridge;
 
-  // Cholesky, one barrier per pivot.  Step k: scale row k-1 (finished in
-  // step k-1, read by nobody from now on), read the pivot d = a_kk, update
-  // rows k+1.. with the unscaled row k times 1/d.
-  const int ti = t >> 4, tj = t & 15;
+  // Blocked right-looking Cholesky.  Panels of 8 pivots are factored row
+  // by row, one thread per column: the pivot and the panel's part of row k
+  // are read, a barrier, then each thread scales its element of row k and
+  // updates the panel rows below it in its column.  The rows past the panel
+  // then take one rank-8 update in 4x4 register tiles (a 16 x 16 thread
+  // grid over the tiles), eight times fewer LDS accesses than updating
+  // after every pivot.  The augmented column f rides along, so it ends as
+  // z = U^{-T} b.
+  const int ta = t >> 4, tc = t & 15;
   bool ok = true;
-  float inv_prev = 0.0f;
-  for (int k = 0; k < f; ++k) {
-    __syncthreads();
-    if (k > 0) {
-      const int rbp = row_base(k - 1, F1);
-      for (int j = k - 1 + t; j < F1; j += kLT) P[rbp + j] *= inv_prev;
+  for (int kb = 0; kb < f; kb += 8) {
+    const int ke = min(kb + 8, f);
+    for (int k = kb; k < ke; ++k) {
+      __syncthreads();
+      const int rbk = row_base(k, F1);
+      const float d = P[rbk + k];
+      if (!(d > 0.0f && d < __builtin_huge_valf())) {  // same value in every thread
+        ok = false;
+        break;
+      }
+      const float inv = rsqrtf(d);
+      float uk[7];
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        const int i = k + 1 + q;
+        uk[q] = i < ke ? P[rbk + i] * inv : 0.0f;
+      }
+      __syncthreads();
+      for (int j = k + t; j < F1; j += kLT) {
+        const float v = P[rbk + j] * inv;
+        P[rbk + j] = v;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+          const int i = k + 1 + q;
+          if (i < ke && i <= j) {
+            const int ri = row_base(i, F1);
+            P[ri + j] = fmaf(-uk[q], v, P[ri + j]);
+          }
+        }
+      }
     }
-    const int rbk = row_base(k, F1);
-    const float d = P[rbk + k];
-    if (!(d > 0.0f && d < __builtin_huge_valf())) {  // same value in every thread
-      ok = false;
-      break;
+    if (!ok || ke >= f) break;  // uniform
+    __syncthreads();            // the panel rows are final
+    const int T = (F1 - ke + 3) >> 2;
+    for (int ti = ta; ti < T; ti += 16) {
+      const int i0 = ke + 4 * ti;
+      if (i0 >= f) break;
+      for (int tj = ti + ((tc - ti) & 15); tj < T; tj += 16) {  // tj >= ti, tj = tc (mod 16)
+        const int j0 = ke + 4 * tj;
+        float acc4[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc4[r][c] = 0.0f;
+        for (int k = kb; k < ke; ++k) {
+          const int rbk = row_base(k, F1);
+          float ui[4], uj[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            ui[r] = P[rbk + min(i0 + r, F1 - 1)];  // clamped reads; their results are not stored
+            uj[r] = P[rbk + min(j0 + r, F1 - 1)];
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc4[r][c] = fmaf(ui[r], uj[c], acc4[r][c]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + r;
+          if (i >= f) continue;
+          const int ri = row_base(i, F1);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int j = j0 + c;
+            if (j >= i && j < F1) P[ri + j] -= acc4[r][c];
+          }
+        }
+      }
     }
-    const float inv = rsqrtf(d);
-    const float inv2 = inv * inv;
-    for (int i = k + 1 + ti; i < f; i += 16) {
-      const float uki = P[rbk + i] * inv2;
-      const int rbi = row_base(i, F1);
-      for (int j = i + tj; j < F1; j += 16) P[rbi + j] = fmaf(-uki, P[rbk + j], P[rbi + j]);
-    }
-    inv_prev = inv;
   }
   __syncthreads();
   if (!ok) {
@@ -162,11 +216,6 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
     for (int i = t; i < (int)a.ldf; i += kLT) xrow[i] = 0.0f;
     return;
   }
-  {
-    const int rbl = row_base(f - 1, F1);
-    for (int j = f - 1 + t; j < F1; j += kLT) P[rbl + j] *= inv_prev;
-  }
-  __syncthreads();
   if (t >= 64) return;
 
   // Back substitution in wavefront 0: entry i of z and x in lane i % 64,
