@@ -47,17 +47,42 @@ __host__ __device__ constexpr int tile_index(int I, int J, int NB) {
 }
 
 // LDS floats per wave: the staging buffers of the Gram phase and the
-// row/column buffers of the solve share the same space.
+// buffers of the Cholesky solve share the same space.
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int chol_lds_floats(int NB) { return NB * 256 + 256 + 32 * NB; }
 __host__ __device__ constexpr int lds_floats_nb(int NB, int f) {
-  return cmax(cmax(2 * KB * ((f + 3) & ~3) + 2 * KB,   // FP32 staging
-                   3 * 32 * 16 * NB / 2),              // BF16 staging (3 x 32 rows x 16NB bf16)
-              cmax(f * (f + 1) / 2 + 2 * f, 32 * NB));  // U, z, z2; two pivot-row buffers
+  return cmax(cmax(2 * KB * ((f + 3) & ~3) + 2 * KB,  // FP32 staging
+                   3 * 32 * 16 * NB / 2),             // BF16 staging (3 x 32 rows x 16NB bf16)
+              chol_lds_floats(NB));                   // W blocks, diagonal tile, z, z2
 }
 
 
 __device__ __forceinline__ float readlane_f(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the 16 lanes of each row group of four per-lane values y0..y3
+// (rows 4g + 0..3): every lane of quad q of the group gets row 4g + q.
+// DPP moves only: row_ror:8 pairs lane c with c ^ 8 (rows {0,1} stay in the
+// low half, {2,3} in the high half), row_half_mirror pairs each lane with one
+// lane of the other quad of its half (each quad keeps one row), two
+// quad_perm steps sum the quad.
+__device__ __forceinline__ float row_reduce_scatter(float y0, float y1, float y2, float y3, int c) {
+  const bool h8 = (c & 8) != 0, h4 = (c & 4) != 0;
+  float k0 = h8 ? y2 : y0, k1 = h8 ? y3 : y1;
+  const float s0 = h8 ? y0 : y2, s1 = h8 ? y1 : y3;
+  k0 += dpp_f<0x128>(s0);  // row_ror:8
+  k1 += dpp_f<0x128>(s1);
+  float k = h4 ? k1 : k0;
+  k += dpp_f<0x141>(h4 ? k0 : k1);  // row_half_mirror
+  k += dpp_f<0x4E>(k);              // quad_perm [2,3,0,1]
+  k += dpp_f<0xB1>(k);              // quad_perm [1,0,3,2]
+  return k;
 }
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -269,179 +294,202 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
   wait_vm_all();
 }
 
-// Right-looking Cholesky on the accumulator layout, software-pipelined by
-// one pivot.  Step k uses the pivot row a_k* (its diagonal d and the values
-// colv[J] = a_k,16J+c were read during step k-1), then:
-//   1. computes row k+1's updated values first (temporaries) and has its
-//      16 owner lanes broadcast them through the other half of a two-row
-//      LDS buffer, and issues the reads of the next pivot and of colv;
-//   2. applies the rank-1 update to every element it owns (the same fused
-//      multiply-adds, so row k+1 ends bit-identical to its broadcast copy).
-// The LDS round trip of pivot k+1 thus overlaps the bulk update of pivot k.
-// K (block row) and KR (register of the pivot row) are compile-time, so
-// every accumulator access is a fixed register; the 16-lane group kg that
-// owns row k is a run-time loop.
-
-// LASTG: kg == 3 is known at compile time, so the pivot after KR == 3 is
-// the first row of block K+1 (a fixed register) rather than of group kg+1.
-// The pivot row's values a_k,16I+4g+{0..3} are streamed block by block
-// (one block in use, one prefetched) to keep the register footprint small.
-template <int NB, int K, int KR, bool LASTG>
-__device__ __forceinline__ void chol_step(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ rb, bool& ok,
-                                          int lane, int kg, float& d, float (&colv)[NB]) {
+// ---------------------------------------------------------------------------
+// Blocked Cholesky on the accumulator tiles, with the O(f^3) work on MFMA.
+//
+// Right-looking over the 16-row blocks K of the augmented Gram:
+//   1. chol_diag: the diagonal tile T is factored by Gauss elimination of
+//      [T | I] run column-wise -- lane c < 16 holds column c of T, lane
+//      16 + c column c of the identity, 16 registers each.  Pivot k scales
+//      row k by 1/sqrt(t_kk) and subtracts u_ki * row k from every row
+//      i > k, the multiplier u_ki read from lane i (v_readlane).  The T half
+//      ends as U_KK (U_KK^T U_KK = T), the identity half as
+//      W_K = U_KK^{-T}.  Pivots at or past f are not taken: those rows of
+//      U and W are zero, so they contribute nothing below.
+//   2. chol_panel: U_KJ = W_K A_KJ for J > K on v_mfma_f32_16x16x4f32.  In
+//      the accumulator layout (lane (g, c), register s holds M[4g+s][c]) a
+//      tile is the B operand of the four k-steps as it stands, with
+//      k = 4g + s; the A operand of step s is W_K[c][4g+s], read from LDS.
+//   3. chol_trailing: A_IJ -= U_KI^T U_KJ for K < I <= J: register s of tile
+//      (K, I), negated, and of tile (K, J) are the A and B operands of step
+//      s, so the update moves no data at all.
+// The augmented column(s) ride along and end as z = U^{-T} b.  Each W_K
+// stays in LDS (column-major) for the back substitution, which runs block
+// by block from the last: x_K = W_K^T (z_K - sum_{J > K} U_KJ x_J).
+//
+// LDS (floats): W_K for every block (NB x 256), the diagonal tile being
+// factored (256), z and the second right-hand side's z (16 NB each).
+// ---------------------------------------------------------------------------
+template <int NB, int K>
+__device__ __forceinline__ void chol_diag(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ wcm,
+                                          float* __restrict__ tsc, bool& ok, int lane) {
   const int g = lane >> 4, c = lane & 15;
-  const int k = 16 * K + 4 * kg + KR;
-  if (k >= f) return;  // uniform
-  const float* cur = rb + (k & 1) * 16 * NB;
-  float* nxt = rb + ((k + 1) & 1) * 16 * NB;
-  f4 rvK = *reinterpret_cast<const f4*>(cur + 16 * K + 4 * g);
-  f4 rvN = {0.f, 0.f, 0.f, 0.f};
-  if constexpr (K + 1 < NB) rvN = *reinterpret_cast<const f4*>(cur + 16 * (K + 1) + 4 * g);
-  const bool good = d > 0.0f && d < __builtin_huge_valf();
-  ok = ok && good;
-  const float inv = good ? __builtin_amdgcn_rsqf(d) : 0.0f;  // 1/sqrt(d)
-  const float inv2 = inv * inv;                               // 1/d
-  float cj[NB];
+  constexpr int TK = tile_index(K, K, NB);
+  // Column-major copy of T: tsc[16 c + i] = T[i][c].
+  *reinterpret_cast<f4*>(tsc + 16 * c + 4 * g) = acc[TK];
+  lds_order();
+  float a[16];
 #pragma unroll
-  for (int J = K; J < NB; ++J) cj[J] = colv[J] * inv2;
-  // Rows of block K at or above k are finished; later blocks are all below k.
+  for (int q = 0; q < 4; ++q) {
+    const f4 v = *reinterpret_cast<const f4*>(tsc + 16 * c + 4 * q);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) rvK[r] = (4 * g + r > 4 * kg + KR) ? rvK[r] : 0.0f;
-  if (g == kg) {  // row k of U
-#pragma unroll
-    for (int J = K; J < NB; ++J) acc[tile_index(K, J, NB)][KR] = colv[J] * inv;
+    for (int r = 0; r < 4; ++r) a[4 * q + r] = lane < 16 ? v[r] : (4 * q + r == c ? 1.0f : 0.0f);
   }
-  if (k + 1 < f) {
-    // Row k+1's owners: same block and group (KR < 3), the next group, or
-    // group 0 of the next block.
-    constexpr int R1 = KR < 3 ? KR + 1 : 0;
-    if constexpr (KR < 3 || !LASTG) {
-      const int g1 = KR < 3 ? kg : kg + 1;
-      if (g == g1) {
+  // Branch-free, so the compiler may interleave this block's VALU work with
+  // the previous block's trailing MFMAs: a pivot at or past f (act false)
+  // scales its row by zero, and zero multipliers leave the other rows alone.
+  const int live = f - 16 * K;  // pivots of this block inside the f x f system
 #pragma unroll
-        for (int J = K; J < NB; ++J) nxt[16 * J + c] = fmaf(-rvK[R1], cj[J], acc[tile_index(K, J, NB)][R1]);
-      }
-    } else if constexpr (K + 1 < NB) {
-      if (g == 0) {
+  for (int k = 0; k < 16; ++k) {
+    const bool act = k < live;
+    const float d = readlane_f(a[k], k);
+    const bool good = d > 0.0f && d < __builtin_huge_valf();
+    ok = ok && (good || !act);
+    const float sk = (act && good) ? __builtin_amdgcn_rsqf(d) : 0.0f;
+    a[k] = act ? a[k] * sk : 0.0f;
+    float u[16];
 #pragma unroll
-        for (int J = K + 1; J < NB; ++J)
-          nxt[16 * J + c] = fmaf(-rvN[0], cj[J], acc[tile_index(K + 1, J, NB)][0]);
-      }
-    }
-    lds_order();
-    d = nxt[k + 1];
+    for (int i = k + 1; i < 16; ++i) u[i] = readlane_f(a[k], i);
 #pragma unroll
-    for (int J = K; J < NB; ++J) colv[J] = nxt[16 * J + c];
+    for (int i = k + 1; i < 16; ++i) a[i] = fmaf(-u[i], a[k], a[i]);
   }
-  // Rank-1 update of every element owned, block row by block row.
-  f4 rv = rvK;  // block I
-  f4 nx = rvN;  // block I+1
+  // U is upper triangular (what elimination leaves below the diagonal is
+  // rounding residue).
 #pragma unroll
-  for (int I = K; I < NB; ++I) {
-    f4 nn = {0.f, 0.f, 0.f, 0.f};  // block I+2
-    if (I + 2 < NB) nn = *reinterpret_cast<const f4*>(cur + 16 * (I + 2) + 4 * g);
+  for (int i = 1; i < 16; ++i) a[i] = (lane < 16 && i > c) ? 0.0f : a[i];
+  lds_order();
+  // U_KK back into tsc (column-major), W_K into its slot; lanes 32..63 hold
+  // the same identity columns as lanes 16..31 and write the same values.
+  float* dst = lane < 16 ? tsc + 16 * c : wcm + 256 * K + 16 * c;
 #pragma unroll
-    for (int J = I; J < NB; ++J)
+  for (int q = 0; q < 4; ++q)
+    *reinterpret_cast<f4*>(dst + 4 * q) = f4{a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]};
+  lds_order();
+  acc[TK] = *reinterpret_cast<const f4*>(tsc + 16 * c + 4 * g);
+}
+
+template <int NB, int K>
+__device__ __forceinline__ void chol_panel(f4 (&acc)[num_tiles(NB)], const float* __restrict__ wcm, int lane) {
+  const int g = lane >> 4, c = lane & 15;
+  const float* w = wcm + 256 * K;
+  f4 wa;
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        acc[tile_index(I, J, NB)][r] = fmaf(-rv[r], cj[J], acc[tile_index(I, J, NB)][r]);
-    rv = nx;
-    nx = nn;
+  for (int s = 0; s < 4; ++s) wa[s] = w[16 * (4 * g + s) + c];  // W_K[c][4g+s]
+#pragma unroll
+  for (int J = K + 1; J < NB; ++J) {
+    const f4 b = acc[tile_index(K, J, NB)];
+    f4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) d = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[s], b[s], d, 0, 0, 0);
+    acc[tile_index(K, J, NB)] = d;
   }
 }
 
 template <int NB, int K>
-__device__ __forceinline__ void chol_block(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ rb, bool& ok,
-                                           int lane, float& d, float (&colv)[NB]) {
-  if (16 * K < f) {
-#pragma unroll 1
-    for (int kg = 0; kg < 3; ++kg) {
-      chol_step<NB, K, 0, false>(acc, f, rb, ok, lane, kg, d, colv);
-      chol_step<NB, K, 1, false>(acc, f, rb, ok, lane, kg, d, colv);
-      chol_step<NB, K, 2, false>(acc, f, rb, ok, lane, kg, d, colv);
-      chol_step<NB, K, 3, false>(acc, f, rb, ok, lane, kg, d, colv);
-    }
-    chol_step<NB, K, 0, true>(acc, f, rb, ok, lane, 3, d, colv);
-    chol_step<NB, K, 1, true>(acc, f, rb, ok, lane, 3, d, colv);
-    chol_step<NB, K, 2, true>(acc, f, rb, ok, lane, 3, d, colv);
-    chol_step<NB, K, 3, true>(acc, f, rb, ok, lane, 3, d, colv);
+__device__ __forceinline__ void chol_trailing(f4 (&acc)[num_tiles(NB)]) {
+#pragma unroll
+  for (int I = K + 1; I < NB; ++I) {
+    const f4 u = acc[tile_index(K, I, NB)];
+    const f4 nu = {-u[0], -u[1], -u[2], -u[3]};
+#pragma unroll
+    for (int J = I; J < NB; ++J)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        acc[tile_index(I, J, NB)] =
+            __builtin_amdgcn_mfma_f32_16x16x4f32(nu[s], acc[tile_index(K, J, NB)][s], acc[tile_index(I, J, NB)], 0, 0, 0);
   }
-  if constexpr (K + 1 < NB) chol_block<NB, K + 1>(acc, f, rb, ok, lane, d, colv);
+}
+
+template <int NB, int K>
+__device__ __forceinline__ void chol_blocks(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ wcm,
+                                            float* __restrict__ tsc, bool& ok, int lane) {
+  // Every block but the last runs unconditionally (one basic block for the
+  // scheduler); the last may hold only the right-hand side column(s).
+  if (K + 1 < NB || 16 * K < f) {
+    chol_diag<NB, K>(acc, f, wcm, tsc, ok, lane);
+    chol_panel<NB, K>(acc, wcm, lane);
+    chol_trailing<NB, K>(acc);
+  }
+  if constexpr (K + 1 < NB) chol_blocks<NB, K + 1>(acc, f, wcm, tsc, ok, lane);
+}
+
+
+template <int NB, int K>
+__device__ __forceinline__ void chol_backsub(const f4 (&acc)[num_tiles(NB)], int f, const float* __restrict__ wcm,
+                                             const float* __restrict__ zb, int lane, float (&xc)[NB]) {
+  const int g = lane >> 4, c = lane & 15;
+  {
+    // Column c of W_K and this lane's z entry first (independent of x).
+    // A block past f has W_K = 0 and gives x_K = 0.
+    const float* w = wcm + 256 * K + 16 * c;
+    const f4 w0 = *reinterpret_cast<const f4*>(w), w1 = *reinterpret_cast<const f4*>(w + 4);
+    const f4 w2 = *reinterpret_cast<const f4*>(w + 8), w3 = *reinterpret_cast<const f4*>(w + 12);
+    const float zv = zb[16 * K + 4 * g + (c >> 2)];
+    float y[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int J = K + 1; J < NB; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) y[r] = fmaf(acc[tile_index(K, J, NB)][r], xc[J], y[r]);
+    // t_i for i = 4g + (c >> 2), so t_j sits in lane 4j.
+    const float t = zv - row_reduce_scatter(y[0], y[1], y[2], y[3], c);
+    const f4 wv[4] = {w0, w1, w2, w3};
+    float x = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x = fmaf(wv[j >> 2][j & 3], readlane_f(t, 4 * j), x);
+    xc[K] = 16 * K + c < f ? x : 0.0f;
+  }
+  if constexpr (K > 0) chol_backsub<NB, K - 1>(acc, f, wcm, zb, lane, xc);
 }
 
 // bcol2 >= 0: a second right-hand-side column whose solution is added to
 // the first (the BF16 low part of the ratings).  xb: optional BF16 copy of
-// the solution (row of ldb entries, zero padded).
+// the solution (row of ldb entries, zero padded).  Returns false (and
+// writes x = 0) if a pivot was not positive.
 template <int NB>
 __device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int bcol,
                                            float* __restrict__ lds, float* __restrict__ xrow,
                                            int64_t ldf, int lane, int bcol2 = -1,
                                            __bf16* __restrict__ xb = nullptr, int64_t ldb = 0) {
   const int g = lane >> 4, c = lane & 15;
+  float* wcm = lds;
+  float* tsc = lds + NB * 256;
+  float* zb = tsc + 256;
+  float* zb2 = zb + 16 * NB;
   bool ok = true;
-  {
-    // Pivot row 0 (group 0, register 0 of tiles (0, J)) into the first buffer.
-    if (g == 0) {
+  chol_blocks<NB, 0>(acc, f, wcm, tsc, ok, lane);
+
+  // z (and the second column's z) out of the augmented column(s), which
+  // sweep_nb places in the last column block (bcol, bcol2 >= 16 (NB - 1)).
+  // Lanes holding other columns store into the free diagonal-tile buffer.
+  const int cb = bcol & 15, cb2 = bcol2 & 15;
+  float* sink = tsc + 4 * lane;
 #pragma unroll
-      for (int J = 0; J < NB; ++J) lds[16 * J + c] = acc[tile_index(0, J, NB)][0];
+  for (int I = 0; I < NB; ++I) {
+    const f4 v = acc[tile_index(I, NB - 1, NB)];
+    *reinterpret_cast<f4*>(c == cb ? zb + 16 * I + 4 * g : sink) = v;
+    if (bcol2 >= 0) *reinterpret_cast<f4*>(c == cb2 ? zb2 + 16 * I + 4 * g : sink) = v;
+  }
+  lds_order();
+  if (bcol2 >= 0) {
+#pragma unroll
+    for (int q = 0; q < (16 * NB + 63) / 64; ++q) {
+      const int i = lane + 64 * q;
+      if (i < 16 * NB) zb[i] += zb2[i];
     }
     lds_order();
-    float d = lds[0];
-    float colv[NB];
-#pragma unroll
-    for (int J = 0; J < NB; ++J) colv[J] = lds[16 * J + c];
-    chol_block<NB, 0>(acc, f, lds, ok, lane, d, colv);
   }
-
-  // Spill U (packed by columns: U[i][j] at j(j+1)/2 + i) and z.
-  const int upk = f * (f + 1) / 2;
-  float* ucol = lds;
-  float* zbuf = lds + upk;
-  float* zbuf2 = zbuf + f;
-  lds_order();
+  float xc[NB];
 #pragma unroll
-  for (int I = 0; I < NB; ++I)
+  for (int J = 0; J < NB; ++J) xc[J] = 0.0f;
+  chol_backsub<NB, NB - 1>(acc, f, wcm, zb, lane, xc);
+  if (g == 0) {
 #pragma unroll
-    for (int J = I; J < NB; ++J) {
-      const int j = 16 * J + c;
-      const int cbase = j * (j + 1) / 2;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * I + 4 * g + r;
-        const float v = acc[tile_index(I, J, NB)][r];
-        if (j < f && i <= j) ucol[cbase + i] = v;
-        if (j == bcol && i < f) zbuf[i] = v;
-        if (j == bcol2 && i < f) zbuf2[i] = v;
-      }
+    for (int K = 0; K < NB; ++K) {
+      const int i = 16 * K + c;
+      const float xv = ok ? xc[K] : 0.0f;
+      if (i < ldf) xrow[i] = xv;
+      if (xb && i < ldb) xb[i] = (__bf16)xv;
     }
-  lds_order();
-
-  // Back substitution, one row index per lane (f <= 192).
-  float z[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int i = lane + 64 * q;
-    z[q] = (i < f) ? zbuf[i] + (bcol2 >= 0 ? zbuf2[i] : 0.0f) : 0.0f;
-  }
-  for (int k = f - 1; k >= 0; --k) {
-    const int kq = k >> 6, kl = k & 63;
-    const float zsel = kq == 0 ? z[0] : (kq == 1 ? z[1] : z[2]);
-    const float zk = readlane_f(zsel, kl);
-    const float xk = zk / ucol[k * (k + 1) / 2 + k];
-    const float* uk = ucol + k * (k + 1) / 2;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const int i = lane + 64 * q;
-      if (i < k) z[q] = fmaf(-uk[i], xk, z[q]);
-      if (i == k) z[q] = xk;
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int i = lane + 64 * q;
-    const float xv = (ok && i < f) ? z[q] : 0.0f;
-    if (i < ldf) xrow[i] = xv;
-    if (xb && i < ldb) xb[i] = (__bf16)xv;
   }
   return ok;
 }
@@ -488,10 +536,6 @@ __host__ __device__ constexpr int cg_lds_floats(int NB, int f) {
 }
 __host__ __device__ constexpr int rows_per_lane(int NB) { return (16 * NB + 63) / 64; }
 
-template <int CTRL>
-__device__ __forceinline__ float dpp_f(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
 // Sum / max over the wave, returned uniform: quad butterflies and two row
 // rotations give every lane its 16-lane row total; the four row totals are
 // combined in a fixed order from lanes 0, 16, 32, 48.
