@@ -19,7 +19,6 @@ Run:  python tests/golden/make_workload_fixtures.py
 import os
 import sys
 import time
-import zlib
 
 import numpy as np
 
@@ -31,13 +30,9 @@ for p in (os.path.join(REPO, "cuda-mllib_amd"), REPO):
 
 import cumf  # noqa: E402
 from oracle import oracle as O  # noqa: E402
+from tests._checks import DATASET_ARRAYS, dataset_crc  # noqa: E402
 
 OUT = os.path.join(HERE, "netflix_f100_fp64.npz")
-ARRAYS = ("csr_row_ptr", "csr_col", "csr_val", "csc_col_ptr", "csc_row", "csc_val", "test_u", "test_v", "test_r")
-
-
-def checksums(ds):
-    return np.array([zlib.crc32(np.ascontiguousarray(getattr(ds, k)).view(np.uint8)) for k in ARRAYS], np.int64)
 
 
 def sample_rows(row_ptr, k, seed):
@@ -68,7 +63,7 @@ def main():
     assert np.array_equal(hist[:2], h2)
     np.savez_compressed(
         OUT, preset=np.array("netflix_480k_17770_99m"), f=np.int64(f), lam=np.float64(lam),
-        iters=np.int64(iters), init_seed=np.int64(0), crc=checksums(ds), crc_arrays=np.array(ARRAYS),
+        iters=np.int64(iters), init_seed=np.int64(0), crc=dataset_crc(ds), crc_arrays=np.array(DATASET_ARRAYS),
         rmse=hist, users=users, items=items, X2=X2[users], T2=T2[items], X10=X10[users], T10=T10[items])
     print(f"wrote {OUT}")
 

@@ -117,3 +117,37 @@ def test_bridge_rejects_bad_lengths_before_touching_the_gpu(env):
     assert res is None
     assert exception(env)[0] == "java/lang/IllegalArgumentException"
     assert FJ.fj_critical_depth(env) == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("f", [10, 100])
+def test_bridge_end_to_end_matches_oracle(env, f):
+    """The Spark entry point's success path: factor blocks flattened, the CSR
+    pinned and uploaded, doALSWithCSR, float[m][f] back -- against the FP64
+    oracle's half-sweep; called twice (the second call reuses the cached
+    task plan) with every critical section released."""
+    import cumf
+    from oracle import oracle as O
+    from tests import _data
+    ds = _data.tiny()
+    lam = 0.05
+    theta = cumf.init_factors(ds.n, f, 3)
+    sizes = [ds.n // 3, 0, ds.n - ds.n // 3 - 7, 7]
+    Xo, _ = O.half_sweep(ds.csr_row_ptr, ds.csr_col, ds.csr_val, theta, lam)
+    for call in range(2):
+        blocks = factor_blocks(env, theta, sizes)
+        rp = np.ascontiguousarray(ds.csr_row_ptr, np.int32)
+        ci = np.ascontiguousarray(ds.csr_col, np.int32)
+        cv = np.ascontiguousarray(ds.csr_val, np.float32)
+        jrp = FJ.fj_new_ints(env, rp.ctypes.data_as(P), rp.shape[0])
+        jci = FJ.fj_new_ints(env, ci.ctypes.data_as(P), ci.shape[0])
+        jcv = FJ.fj_new_floats(env, cv.ctypes.data_as(P), cv.shape[0])
+        res = JNI_DO(FJ.fj_env(env), None, ds.m, ds.n, f, ds.nnz, lam, blocks, jrp, jci, jcv)
+        assert exception(env) is None
+        assert res and FJ.fj_len(res) == ds.m
+        assert FJ.fj_critical_depth(env) == 0
+        X = np.zeros((ds.m, f), np.float32)
+        for u in range(ds.m):
+            assert FJ.fj_floats(FJ.fj_get(res, u), X[u].ctypes.data_as(P), f) == f
+        rel = np.linalg.norm(X - Xo, axis=1) / np.maximum(np.linalg.norm(Xo, axis=1), 1e-30)
+        assert rel.max() <= 1e-4, (call, rel.max())
