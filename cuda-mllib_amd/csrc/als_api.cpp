@@ -41,7 +41,7 @@ constexpr int kMaxFusedF = 140;
 constexpr int kDefaultSplit = 16384;
 constexpr int kDefaultCgIters = 6;
 constexpr int kDefaultCgRounds = 6;
-constexpr float kDefaultCgTol = 1e-3f;
+constexpr float kDefaultCgTol = 1e-5f;  // the per-row relative residual bar
 
 int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }
 
@@ -51,11 +51,14 @@ constexpr int kBuckets = 8;
 constexpr int64_t kBucketEdge[kBuckets + 1] = {0, 16, 64, 256, 1024, 4096, 16384, 65536, INT64_MAX};
 
 // Launch shape of one bucket: register budget (wavefronts per SIMD the
-// kernel was compiled for, 1 or 2) and persistent grid (0: one work-group
-// per task; > 0: that many wavefronts taking tasks from a counter).
+// kernel was compiled for, 1 or 2), persistent grid (0: one work-group per
+// task; > 0: that many wavefronts taking tasks from a counter) and solver
+// (-1: the handle's; CUMF_SOLVER_CHOLESKY or CUMF_SOLVER_CG, which the
+// autotuner may pick per bucket when the handle's solver is CG).
 struct BucketCfg {
   int variant = 2;
   int persistent = 0;
+  int solver = -1;
 };
 
 // Wavefront task list of one rating matrix side.  Tasks are ordered by
@@ -67,7 +70,9 @@ struct Plan {
   DevBuf tasks, heavy, partial;
 
   int build(const int32_t* row_ptr, int64_t m, int split_, int f) {
-    const int64_t split = split_;
+    // Above kMaxFusedF the work-group-per-row kernel solves whole rows:
+    // nothing is split and no partial Grams are needed.
+    const int64_t split = f <= kMaxFusedF ? split_ : INT64_MAX;
     std::vector<cumf::Task> t;
     std::vector<cumf::HeavyRow> hv;
     t.reserve((size_t)m);
@@ -93,7 +98,7 @@ struct Plan {
                      [](const cumf::Task& a, const cumf::Task& b) { return a.count > b.count; });
     rows = m;
     nnz = row_ptr[m];
-    this->split = (int)split;
+    this->split = f <= kMaxFusedF ? (int)split : 0;
     ntasks = (int64_t)t.size();
     for (int b = 0; b < kBuckets; ++b) bucket_t0[b] = bucket_t1[b] = 0;
     for (int64_t i = 0; i < ntasks; ++i) {
@@ -270,6 +275,7 @@ hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::Sw
   x.ntasks = p.bucket_t1[b] - p.bucket_t0[b];
   x.persistent = cfg.persistent;
   x.task_counter = h->counters.as<int>() + side * kBuckets + b;
+  if (cfg.solver >= 0) x.solver = cfg.solver;
   return task_launch(x, cfg.variant, h->stream);
 }
 
@@ -280,15 +286,23 @@ int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs&
   if (!h->tiled) return timed(h, Timer::TASK, [&] { return task_launch(a, default_variant(h->f), h->stream); });
   // A side whose non-empty buckets share one shape and no persistent grid is
   // one launch (launches in sequence drain the GPU between buckets).
-  int v0 = 0;
+  int v0 = 0, s0 = -2;
   bool uniform = true;
   for (int b = 0; b < kBuckets; ++b) {
     if (p.bucket_t1[b] <= p.bucket_t0[b]) continue;
     const BucketCfg& c = h->tiles[side][b];
-    if (v0 == 0) v0 = c.variant;
-    uniform = uniform && c.persistent == 0 && c.variant == v0;
+    const int sv = c.solver >= 0 ? c.solver : a.solver;
+    if (v0 == 0) {
+      v0 = c.variant;
+      s0 = sv;
+    }
+    uniform = uniform && c.persistent == 0 && c.variant == v0 && sv == s0;
   }
-  if (uniform && v0 != 0) return timed(h, Timer::TASK, [&] { return task_launch(a, v0, h->stream); });
+  if (uniform && v0 != 0) {
+    cumf::SweepArgs x = a;
+    x.solver = s0;
+    return timed(h, Timer::TASK, [&] { return task_launch(x, v0, h->stream); });
+  }
   bool persistent = false;
   for (int b = 0; b < kBuckets; ++b) persistent = persistent || h->tiles[side][b].persistent > 0;
   if (persistent)
@@ -944,9 +958,12 @@ int rebuild_plans(cumf_handle* h) {
   return CUMF_OK;
 }
 
-// Minimum over reps of the stream time of launch() (milliseconds).
-template <class F>
-int time_min_ms(cumf_handle* h, int reps, F&& launch, double* out) {
+// Minimum over reps of the stream time of launch() (milliseconds); prep()
+// runs before each rep, outside the timed interval (it puts back the
+// factors the timed sweeps overwrite, so that every rep solves from the
+// same start).
+template <class P, class F>
+int time_min_ms(cumf_handle* h, int reps, P&& prep, F&& launch, double* out) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   CUMF_HIP_TRY(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -956,6 +973,8 @@ int time_min_ms(cumf_handle* h, int reps, F&& launch, double* out) {
   double best = 1e300;
   int rc = CUMF_OK;
   for (int r = 0; r < reps && rc == CUMF_OK; ++r) {
+    rc = prep();
+    if (rc != CUMF_OK) break;
     (void)hipEventRecord(e0, h->stream);
     rc = launch();
     (void)hipEventRecord(e1, h->stream);
@@ -983,11 +1002,27 @@ SideBufs side_bufs(cumf_handle* h, int side) {
   return {&h->plan_t, &h->cp, &h->ri, &h->cv, h->X.as<float>(), h->T.as<float>(), &h->Xb, &h->Tb};
 }
 
-int autotune_side(cumf_handle* h, int side, int reps) {
+int copy_dev(DevBuf& dst, const DevBuf& src, hipStream_t s);
+
+// The factors a side's timed sweeps overwrite, as they were when tuning
+// started (FP32 and, in precision mode, BF16).
+struct TuneStart {
+  const DevBuf* x;
+  const DevBuf* xb;
+};
+
+int autotune_side(cumf_handle* h, int side, int reps, const TuneStart& start) {
   const std::vector<int32_t>& rp = side == 0 ? h->rp_host : h->cp_host;
   const int64_t rows = side == 0 ? h->m : h->n;
   int64_t maxdeg = 0;
   for (int64_t u = 0; u < rows; ++u) maxdeg = std::max<int64_t>(maxdeg, (int64_t)rp[u + 1] - rp[u]);
+  DevBuf& xo = side == 0 ? h->X : h->T;
+  DevBuf& xbo = side == 0 ? h->Xb : h->Tb;
+  auto restore = [&] {
+    CUMF_TRY(copy_dev(xo, *start.x, h->stream));
+    if (start.xb->p) CUMF_TRY(copy_dev(xbo, *start.xb, h->stream));
+    return CUMF_OK;
+  };
   // 1. Split size: whole half-sweeps, one launch per kernel.
   h->tiled = false;
   static const int kSplits[] = {1024, 2048, 4096, 8192, 16384, 32768};
@@ -998,7 +1033,7 @@ int autotune_side(cumf_handle* h, int side, int reps) {
     h->tile_split[side] = sp;
     CUMF_TRY(rebuild_plans(h));
     double ms = 0;
-    CUMF_TRY(time_min_ms(h, reps, [&] {
+    CUMF_TRY(time_min_ms(h, reps, restore, [&] {
       SideBufs b = side_bufs(h, side);
       return run_half_sweep(h, side, *b.p, *b.rowp, *b.col, *b.val, b.theta, b.x, *b.theta_b, *b.x_b);
     }, &ms));
@@ -1009,31 +1044,37 @@ int autotune_side(cumf_handle* h, int side, int reps) {
   }
   h->tile_split[side] = best_split;
   CUMF_TRY(rebuild_plans(h));
-  // 2. Launch shape per degree bucket.
+  // 2. Launch shape per degree bucket: register budget x persistent grid,
+  //    and on a CG handle the solver (a CG bucket is timed with the
+  //    Cholesky re-solve of the rows it hands back).
   SideBufs b = side_bufs(h, side);
   cumf::SweepArgs a{};
   CUMF_TRY(fill_args(h, *b.p, *b.rowp, *b.col, *b.val, b.theta, b.x, *b.theta_b, *b.x_b, nullptr, a));
   a.warm = side == 0 ? std::min(h->opt.warm_start, (int)CUMF_WARM_ZERO) : CUMF_WARM_PREV;
   const bool cg = h->opt.solver == CUMF_SOLVER_CG;
+  const int dv = default_variant(h->f);
+  std::vector<int> solvers = cg ? std::vector<int>{CUMF_SOLVER_CG, CUMF_SOLVER_CHOLESKY} : std::vector<int>{-1};
   for (int k = 0; k < kBuckets; ++k) {
     if (b.p->bucket_t1[k] <= b.p->bucket_t0[k]) continue;
     double bk = 1e300;
     BucketCfg bc;
-    for (int variant : {2, 1})
-      for (int pers : {0, h->num_cus * (variant == 1 ? 4 : 8)}) {
-        BucketCfg cfg{variant, pers};
-        double ms = 0;
-        CUMF_TRY(time_min_ms(h, reps, [&] {
-          if (cg) CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
-          if (pers) CUMF_HIP_TRY(cumf::launch_fallback_reset(h->counters.as<int>() + side * kBuckets + k, 1, h->stream));
-          CUMF_HIP_TRY(bucket_launch(h, side, *b.p, a, k, cfg));
-          return CUMF_OK;
-        }, &ms));
-        if (ms < bk) {
-          bk = ms;
-          bc = cfg;
+    for (int sol : solvers)
+      for (int variant : {2, 1})
+        for (int pers : {0, h->num_cus * (variant == 1 ? 4 : 8)}) {
+          BucketCfg cfg{variant, pers, sol};
+          double ms = 0;
+          CUMF_TRY(time_min_ms(h, reps, restore, [&] {
+            if (cg) CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
+            if (pers) CUMF_HIP_TRY(cumf::launch_fallback_reset(h->counters.as<int>() + side * kBuckets + k, 1, h->stream));
+            CUMF_HIP_TRY(bucket_launch(h, side, *b.p, a, k, cfg));
+            if (cg) CUMF_HIP_TRY(fallback_launch(a, dv, h->stream));
+            return CUMF_OK;
+          }, &ms));
+          if (ms < bk) {
+            bk = ms;
+            bc = cfg;
+          }
         }
-      }
     h->tiles[side][k] = bc;
   }
   // 3. Register budget of the split-row finish and of the Cholesky fallback
@@ -1048,8 +1089,8 @@ int autotune_side(cumf_handle* h, int side, int reps) {
         return CUMF_OK;
       };
     };
-    CUMF_TRY(time_min_ms(h, reps, fin(2), &t2));
-    CUMF_TRY(time_min_ms(h, reps, fin(1), &t1));
+    CUMF_TRY(time_min_ms(h, reps, restore, fin(2), &t2));
+    CUMF_TRY(time_min_ms(h, reps, restore, fin(1), &t1));
     h->finish_variant[side] = t1 < t2 ? 1 : 2;
   }
   h->fallback_variant[side] = 2;
@@ -1070,15 +1111,16 @@ int autotune_side(cumf_handle* h, int side, int reps) {
           return CUMF_OK;
         };
       };
-      CUMF_TRY(time_min_ms(h, reps, fb(2), &t2));
-      CUMF_TRY(time_min_ms(h, reps, fb(1), &t1));
+      CUMF_TRY(time_min_ms(h, reps, restore, fb(2), &t2));
+      CUMF_TRY(time_min_ms(h, reps, restore, fb(1), &t1));
       h->fallback_variant[side] = t1 < t2 ? 1 : 2;
       CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
     }
   }
   // 4. Buckets launched one after another drain the GPU between launches:
   //    keep the per-bucket shapes only if a whole half-sweep with them beats
-  //    one launch of every task with the default shape.
+  //    one launch of every task with the default shape and the handle's
+  //    solver.
   {
     BucketCfg tuned[kBuckets];
     std::copy(h->tiles[side], h->tiles[side] + kBuckets, tuned);
@@ -1089,9 +1131,9 @@ int autotune_side(cumf_handle* h, int side, int reps) {
       return run_half_sweep(h, side, *sb.p, *sb.rowp, *sb.col, *sb.val, sb.theta, sb.x, *sb.theta_b, *sb.x_b);
     };
     double t_buckets = 0, t_single = 0;
-    CUMF_TRY(time_min_ms(h, reps, sweep, &t_buckets));
-    for (int k = 0; k < kBuckets; ++k) h->tiles[side][k] = BucketCfg{default_variant(h->f), 0};
-    CUMF_TRY(time_min_ms(h, reps, sweep, &t_single));
+    CUMF_TRY(time_min_ms(h, reps, restore, sweep, &t_buckets));
+    for (int k = 0; k < kBuckets; ++k) h->tiles[side][k] = BucketCfg{dv, 0, -1};
+    CUMF_TRY(time_min_ms(h, reps, restore, sweep, &t_single));
     if (t_buckets < t_single) std::copy(tuned, tuned + kBuckets, h->tiles[side]);
     h->tiled = was_tiled;
   }
@@ -1119,15 +1161,16 @@ int cumf_save_tile_table(cumf_handle* h, const char* path) {
   }
   std::fprintf(fp, "# cumf tile table: f %d precision %d solver %d\n", h->f, h->opt.precision, h->opt.solver);
   std::fprintf(fp, "# side <0 X | 1 Theta> split <ratings per task> finish <1|2> fallback <1|2>\n");
-  std::fprintf(fp, "# bucket <side> <b> <min ratings> <max ratings> variant <1|2> persistent <wavefronts>\n");
+  std::fprintf(fp, "# bucket <side> <b> <min ratings> <max ratings> variant <1|2> persistent <wavefronts>"
+                  " solver <-1 handle's | 0 Cholesky | 1 CG>\n");
   for (int side = 0; side < 2; ++side) {
     const Plan& p = side == 0 ? h->plan_x : h->plan_t;
     std::fprintf(fp, "side %d split %d finish %d fallback %d\n", side, p.split > 0 ? p.split : h->opt.split,
                  h->finish_variant[side], h->fallback_variant[side]);
     for (int b = 0; b < kBuckets; ++b)
-      std::fprintf(fp, "bucket %d %d %lld %lld variant %d persistent %d\n", side, b, (long long)kBucketEdge[b],
-                   (long long)(b + 1 < kBuckets ? kBucketEdge[b + 1] - 1 : -1), h->tiles[side][b].variant,
-                   h->tiles[side][b].persistent);
+      std::fprintf(fp, "bucket %d %d %lld %lld variant %d persistent %d solver %d\n", side, b,
+                   (long long)kBucketEdge[b], (long long)(b + 1 < kBuckets ? kBucketEdge[b + 1] - 1 : -1),
+                   h->tiles[side][b].variant, h->tiles[side][b].persistent, h->tiles[side][b].solver);
   }
   const bool ok = std::fclose(fp) == 0;
   if (!ok) cumf::set_last_error(std::string("cannot write ") + path);
@@ -1149,16 +1192,20 @@ int cumf_load_tile_table(cumf_handle* h, const char* path) {
   while (rc == CUMF_OK && std::fgets(line, sizeof line, fp)) {
     ++lineno;
     if (line[0] == '#' || line[0] == '\n') continue;
-    int side, sp, fv, bv, b, v, pers;
+    int side, sp, fv, bv, b, v, pers, sol = -1, nf = 0;
     long long lo, hi;
     if (std::sscanf(line, "side %d split %d finish %d fallback %d", &side, &sp, &fv, &bv) == 4) {
       if (side < 0 || side > 1 || sp < 1 || (fv != 1 && fv != 2) || (bv != 1 && bv != 2)) rc = CUMF_ERR_INVALID;
       else { split[side] = sp; fin[side] = fv; fb[side] = bv; }
-    } else if (std::sscanf(line, "bucket %d %d %lld %lld variant %d persistent %d", &side, &b, &lo, &hi, &v,
-                           &pers) == 6) {
-      if (side < 0 || side > 1 || b < 0 || b >= kBuckets || (v != 1 && v != 2) || pers < 0 || pers > (1 << 20))
+    } else if ((nf = std::sscanf(line, "bucket %d %d %lld %lld variant %d persistent %d solver %d", &side, &b, &lo,
+                                 &hi, &v, &pers, &sol)) >= 6) {
+      // A bucket may run the CG solver only on a handle built for CG (its
+      // fallback list and kernel); Cholesky is always available.
+      if (nf == 6) sol = -1;
+      if (side < 0 || side > 1 || b < 0 || b >= kBuckets || (v != 1 && v != 2) || pers < 0 || pers > (1 << 20) ||
+          sol < -1 || sol > 1 || (sol == CUMF_SOLVER_CG && h->opt.solver != CUMF_SOLVER_CG))
         rc = CUMF_ERR_INVALID;
-      else tiles[side][b] = {v, pers};
+      else tiles[side][b] = {v, pers, sol};
     } else {
       rc = CUMF_ERR_INVALID;
     }
@@ -1209,15 +1256,20 @@ int cumf_autotune(cumf_handle* h, int reps, const char* path) {
   const int warm = h->opt.warm_start;
   h->opt.timing = 0;
   if (warm >= CUMF_WARM_MLP) h->opt.warm_start = CUMF_WARM_PREV;  // the network is not part of the tile choice
-  int rc = autotune_side(h, 0, reps);
-  if (rc == CUMF_OK) rc = autotune_side(h, 1, reps);
+  int rc = autotune_side(h, 0, reps, TuneStart{&sx, &sxb});
+  if (rc == CUMF_OK) rc = autotune_side(h, 1, reps, TuneStart{&st, &stb});
   h->opt.timing = timing;
   h->opt.warm_start = warm;
-  if (rc == CUMF_OK) rc = copy_dev(h->X, sx, h->stream);
-  if (rc == CUMF_OK) rc = copy_dev(h->T, st, h->stream);
-  if (rc == CUMF_OK && sxb.p) rc = copy_dev(h->Xb, sxb, h->stream);
-  if (rc == CUMF_OK && stb.p) rc = copy_dev(h->Tb, stb, h->stream);
-  if (rc == CUMF_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = CUMF_ERR_HIP;
+  // The factors are put back whether or not tuning succeeded; the first
+  // error is the one reported.
+  auto keep_first = [&rc](int r) {
+    if (rc == CUMF_OK) rc = r;
+  };
+  keep_first(copy_dev(h->X, sx, h->stream));
+  keep_first(copy_dev(h->T, st, h->stream));
+  if (sxb.p) keep_first(copy_dev(h->Xb, sxb, h->stream));
+  if (stb.p) keep_first(copy_dev(h->Tb, stb, h->stream));
+  if (hipStreamSynchronize(h->stream) != hipSuccess) keep_first(CUMF_ERR_HIP);
   // Counters of the tuning runs are not statistics of the user's sweeps.
   if (rc == CUMF_OK && hipMemset(h->fail.p, 0, 2 * sizeof(int)) != hipSuccess) rc = CUMF_ERR_HIP;
   h->tiled = rc == CUMF_OK;
@@ -1456,13 +1508,61 @@ int cumf_debug_copy(int device, int64_t bytes, int iters, double* ms_per_launch)
   return CUMF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Per-device state the device-pointer half-sweep keeps between calls: the
+// task plan of the last rating matrix, reused when the next call brings the
+// same row pointers (Spark calls the drop-in once per half-sweep on the
+// same blocks), and device buffers kept at the largest size asked for so
+// far.  A call holds its device's cache for its whole duration; a
+// concurrent call on the same device that finds it held works with buffers
+// of its own.  The caches live until the process ends.
+struct SweepCache {
+  std::mutex mu;
+  std::vector<int32_t> rp;  // row pointers the plan was built from
+  int f = 0, split = 0;
+  Plan plan;
+  DevBuf theta, x, tpad, xpad, fail;
+};
+
+SweepCache* sweep_cache(int dev) {
+  static std::mutex mu;
+  static std::vector<SweepCache*>* caches = new std::vector<SweepCache*>();
+  if (dev < 0) return nullptr;
+  std::lock_guard<std::mutex> g(mu);
+  if ((size_t)dev >= caches->size()) caches->resize((size_t)dev + 1, nullptr);
+  if (!(*caches)[dev]) (*caches)[dev] = new SweepCache();
+  return (*caches)[dev];
+}
+
+// Holds the current device's cache if it is free.
+struct CacheLease {
+  SweepCache* c = nullptr;
+  std::unique_lock<std::mutex> lk;
+  CacheLease() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    SweepCache* sc = sweep_cache(dev);
+    if (!sc) return;
+    lk = std::unique_lock<std::mutex>(sc->mu, std::try_to_lock);
+    if (lk.owns_lock()) c = sc;
+  }
+};
+
+int ensure(DevBuf& b, size_t bytes) { return b.bytes >= bytes ? CUMF_OK : b.alloc(bytes); }
+
 // Device-pointer half-sweep.  The CSR row pointers are read back to plan
-// the tasks; factor buffers with f % 4 != 0 are re-strided through padded
-// temporaries (the kernels read factor rows with 16-byte loads).
-int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const int32_t* d_col,
-                           const float* d_val, const float* d_theta, float* d_x, int64_t m,
-                           int64_t n, int f, int64_t nnz, float lambda, const cumf_options* opt) {
-  cumf::clear_last_error();
+// the tasks (or to recognise the cached plan); factor buffers with
+// f % 4 != 0 are re-strided through padded temporaries (the kernels read
+// factor rows with 16-byte loads).
+int half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const int32_t* d_col, const float* d_val,
+                      const float* d_theta, float* d_x, int64_t m, int64_t n, int f, int64_t nnz, float lambda,
+                      const cumf_options* opt, SweepCache* cache) {
   CUMF_TRY(check_f(f));
   if (m <= 0 || n <= 0 || nnz < 0 || nnz > INT32_MAX || m > INT32_MAX || !d_row_ptr || !d_theta ||
       !d_x || (nnz > 0 && (!d_col || !d_val)) || !(lambda >= 0.f)) {
@@ -1488,23 +1588,41 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
     cumf::set_last_error("row pointers must start at 0 and end at nnz");
     return CUMF_ERR_INVALID;
   }
-  Plan plan;
-  CUMF_TRY(plan.build(rp.data(), m, o.split, f));
+  Plan local_plan;
+  DevBuf local_tpad, local_xpad, local_fail;
+  Plan* plan = &local_plan;
+  DevBuf* tpad = &local_tpad;
+  DevBuf* xpad = &local_xpad;
+  DevBuf* fail = &local_fail;
+  if (cache) {
+    plan = &cache->plan;
+    tpad = &cache->tpad;
+    xpad = &cache->xpad;
+    fail = &cache->fail;
+    if (!(cache->f == f && cache->split == o.split && cache->rp == rp)) {
+      cache->rp.clear();  // no stale match if the build fails
+      CUMF_TRY(cache->plan.build(rp.data(), m, o.split, f));
+      cache->rp.swap(rp);
+      cache->f = f;
+      cache->split = o.split;
+    }
+  } else {
+    CUMF_TRY(local_plan.build(rp.data(), m, o.split, f));
+  }
   const int64_t ldf = round_up4(f);
-  DevBuf tpad, xpad, fail;
   const float* theta = d_theta;
   float* x = d_x;
   if (ldf != f) {
-    CUMF_TRY(tpad.alloc(sizeof(float) * n * ldf));
-    CUMF_TRY(xpad.alloc(sizeof(float) * m * ldf));
-    CUMF_HIP_TRY(hipMemsetAsync(tpad.p, 0, tpad.bytes, stream));
-    CUMF_HIP_TRY(hipMemcpy2DAsync(tpad.p, sizeof(float) * ldf, d_theta, sizeof(float) * f,
+    CUMF_TRY(ensure(*tpad, sizeof(float) * n * ldf));
+    CUMF_TRY(ensure(*xpad, sizeof(float) * m * ldf));
+    CUMF_HIP_TRY(hipMemsetAsync(tpad->p, 0, sizeof(float) * n * ldf, stream));
+    CUMF_HIP_TRY(hipMemcpy2DAsync(tpad->p, sizeof(float) * ldf, d_theta, sizeof(float) * f,
                                   sizeof(float) * f, n, hipMemcpyDeviceToDevice, stream));
-    theta = tpad.as<float>();
-    x = xpad.as<float>();
+    theta = tpad->as<float>();
+    x = xpad->as<float>();
   }
-  CUMF_TRY(fail.alloc(sizeof(int)));
-  CUMF_HIP_TRY(hipMemsetAsync(fail.p, 0, sizeof(int), stream));
+  CUMF_TRY(ensure(*fail, sizeof(int)));
+  CUMF_HIP_TRY(hipMemsetAsync(fail->p, 0, sizeof(int), stream));
   cumf::SweepArgs a{};
   a.row_ptr = d_row_ptr;
   a.col = d_col;
@@ -1512,12 +1630,12 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
   a.theta = theta;
   a.x = x;
   a.ldf = ldf;
-  a.tasks = plan.tasks.as<cumf::Task>();
-  a.ntasks = plan.ntasks;
-  a.heavy = plan.heavy.as<cumf::HeavyRow>();
-  a.nheavy = plan.nheavy;
-  a.partial = plan.partial.as<float>();
-  a.fail_counter = fail.as<int>();
+  a.tasks = plan->tasks.as<cumf::Task>();
+  a.ntasks = plan->ntasks;
+  a.heavy = plan->heavy.as<cumf::HeavyRow>();
+  a.nheavy = plan->nheavy;
+  a.partial = plan->partial.as<float>();
+  a.fail_counter = fail->as<int>();
   a.f = f;
   a.lambda = lambda;
   if (o.compat_hermitian_bug) {
@@ -1528,7 +1646,7 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
     c.theta = theta;
     c.x = x;
     c.ldf = ldf;
-    c.fail_counter = fail.as<int>();
+    c.fail_counter = fail->as<int>();
     c.f = f;
     c.lambda = lambda;
     CUMF_HIP_TRY(cumf::prepare_compat(f));
@@ -1547,6 +1665,18 @@ int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const i
   return CUMF_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const int32_t* d_col,
+                           const float* d_val, const float* d_theta, float* d_x, int64_t m,
+                           int64_t n, int f, int64_t nnz, float lambda, const cumf_options* opt) {
+  cumf::clear_last_error();
+  CacheLease lease;
+  return half_sweep_device(stream, d_row_ptr, d_col, d_val, d_theta, d_x, m, n, f, nnz, lambda, opt, lease.c);
+}
+
 // Drop-in for als/src/cuda/als.cu:1000-1115 (see include/als.h).
 void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, float* csrVal,
                   float* thetaTHost, float* XTHost, const int m, const int n, const int f,
@@ -1559,17 +1689,23 @@ void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, f
       return CUMF_ERR_INVALID;
     }
     CUMF_TRY(check_f(f));
-    DevBuf theta, x;
-    CUMF_TRY(theta.alloc(sizeof(float) * (size_t)n * f));
-    CUMF_TRY(x.alloc(sizeof(float) * (size_t)m * f));
-    CUMF_HIP_TRY(hipMemcpyAsync(theta.p, thetaTHost, theta.bytes, hipMemcpyHostToDevice, cuda_stream));
+    // The factor buffers and the task plan come from the device's cache
+    // when it is free (see SweepCache), else from allocations of this call.
+    CacheLease lease;
+    DevBuf local_theta, local_x;
+    DevBuf& theta = lease.c ? lease.c->theta : local_theta;
+    DevBuf& x = lease.c ? lease.c->x : local_x;
+    const size_t tb = sizeof(float) * (size_t)n * f, xb = sizeof(float) * (size_t)m * f;
+    CUMF_TRY(ensure(theta, tb));
+    CUMF_TRY(ensure(x, xb));
+    CUMF_HIP_TRY(hipMemcpyAsync(theta.p, thetaTHost, tb, hipMemcpyHostToDevice, cuda_stream));
     cumf_options o;
     cumf_default_options(&o);
     const char* env = std::getenv("CUMF_COMPAT_HERMITIAN_BUG");
     o.compat_hermitian_bug = (env && std::strcmp(env, "1") == 0) ? 1 : 0;
-    CUMF_TRY(cumf_half_sweep_device(cuda_stream, csrRowIndex, csrColIndex, csrVal, theta.as<float>(),
-                                    x.as<float>(), m, n, f, nnz, lambda, &o));
-    CUMF_HIP_TRY(hipMemcpyAsync(XTHost, x.p, x.bytes, hipMemcpyDeviceToHost, cuda_stream));
+    CUMF_TRY(half_sweep_device(cuda_stream, csrRowIndex, csrColIndex, csrVal, theta.as<float>(), x.as<float>(),
+                               m, n, f, nnz, lambda, &o, lease.c));
+    CUMF_HIP_TRY(hipMemcpyAsync(XTHost, x.p, xb, hipMemcpyDeviceToHost, cuda_stream));
     CUMF_HIP_TRY(hipStreamSynchronize(cuda_stream));
     return CUMF_OK;
   };

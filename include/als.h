@@ -111,7 +111,9 @@ typedef struct cumf_options {
   int32_t solver;       /* CUMF_SOLVER_* */
   int32_t cg_iters;     /* CG steps per refinement round (0: 6) */
   int32_t cg_rounds;    /* refinement rounds, at most (0: 6) */
-  float cg_tol;         /* accept ||b - A x|| <= cg_tol ||b|| (0: 1e-3) */
+  float cg_tol;         /* accept ||b - A x|| <= cg_tol ||b||, the residual
+                           evaluated in FP64 on the Gram the GPU formed
+                           (0: 1e-5) */
   int32_t warm_start;   /* CUMF_WARM_* (CG only) */
   int32_t reserved[5];
 } cumf_options;
@@ -192,6 +194,19 @@ int cumf_load_csr_bin(const char* dataFile, const char* rowFile, const char* col
                       int* row, int* col, int m, long nnz);
 int cumf_save_csr_bin(const char* dataFile, const char* rowFile, const char* colFile,
                       const float* data, const int* row, const int* col, int m, long nnz);
+/* Seeded held-out split of a CSR rating matrix (host arrays), for ingested
+ * files that come without a test set; no counterpart in the reference (its
+ * Spark caller splits before the GPU sees the data).  Each rating is held
+ * out with probability `fraction`, decided by a hash of (seed, rating
+ * index); every row and every column with ratings keeps at least one
+ * training rating.  Writes the training CSR (train_row_ptr[m+1],
+ * train_col/train_val) and the held-out COO (test_u/v/r), order preserved;
+ * every output array except train_row_ptr must hold nnz entries.
+ * *train_nnz + *test_nnz == nnz. */
+int cumf_split_holdout(int64_t m, int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
+                       const float* val, double fraction, uint64_t seed, int32_t* train_row_ptr,
+                       int32_t* train_col, float* train_val, int64_t* train_nnz, int32_t* test_u,
+                       int32_t* test_v, float* test_r, int64_t* test_nnz);
 /* CSR (m x n, device arrays) -> CSC (col_ptr n+1, row nnz, val nnz, device
  * arrays) by a stable radix sort on the device: rows stay ascending inside
  * each column, so the result equals a host counting-sort transpose.  No
