@@ -117,6 +117,8 @@ def lib():
         "get_gpu": ([], C.c_int),
         "cumf_load_csr_bin": ([C.c_char_p, C.c_char_p, C.c_char_p, P, P, P, C.c_int, C.c_long], C.c_int),
         "cumf_save_csr_bin": ([C.c_char_p, C.c_char_p, C.c_char_p, P, P, P, C.c_int, C.c_long], C.c_int),
+        "cumf_split_holdout": ([C.c_int64, C.c_int64, C.c_int64, P, P, P, C.c_double, C.c_uint64, P, P, P,
+                                C.POINTER(C.c_int64), P, P, P, C.POINTER(C.c_int64)], C.c_int),
         "doALSWithCSR": ([P, P, P, P, P, P, C.c_int, C.c_int, C.c_int, C.c_long, C.c_float, C.c_int],
                          None),
     }
@@ -384,3 +386,21 @@ def load_csr_bin(prefix, m, nnz):
                                    f"{prefix}.col.bin".encode(), _ptr(cv), _ptr(rp), _ptr(ci), m, nnz),
            "cumf_load_csr_bin")
     return rp, ci, cv
+
+
+def split_holdout(row_ptr, col, val, n, fraction, seed=0):
+    """Seeded held-out split (cumf_split_holdout): returns the training CSR
+    (row_ptr, col, val) and the held-out COO (u, v, r)."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    ci = np.ascontiguousarray(col, np.int32)
+    cv = np.ascontiguousarray(val, np.float32)
+    m, nnz = rp.shape[0] - 1, ci.shape[0]
+    trp = np.empty(m + 1, np.int32)
+    tci, tcv = np.empty(nnz, np.int32), np.empty(nnz, np.float32)
+    hu, hv, hr = np.empty(nnz, np.int32), np.empty(nnz, np.int32), np.empty(nnz, np.float32)
+    nt, nh = C.c_int64(), C.c_int64()
+    _check(lib().cumf_split_holdout(m, int(n), nnz, _ptr(rp), _ptr(ci), _ptr(cv), float(fraction), int(seed),
+                                    _ptr(trp), _ptr(tci), _ptr(tcv), C.byref(nt), _ptr(hu), _ptr(hv), _ptr(hr),
+                                    C.byref(nh)), "cumf_split_holdout")
+    a, b = nt.value, nh.value
+    return trp, tci[:a].copy(), tcv[:a].copy(), hu[:b].copy(), hv[:b].copy(), hr[:b].copy()
