@@ -61,8 +61,42 @@ struct BucketCfg {
   int solver = -1;
 };
 
-// Wavefront task list of one rating matrix side.  Tasks are ordered by
-// decreasing rating count, so every degree bucket is a contiguous range.
+// Host half of the planner: every row with at most `split` ratings is one
+// task; a longer row is cut into split-sized chunks, each a task writing
+// its partial Gram to a scratch slot (consecutive slots per row), and the
+// row is listed as heavy.  Above kMaxFusedF the work-group-per-row kernel
+// solves whole rows: nothing is split.  Tasks are ordered by decreasing
+// rating count (stable), so every degree bucket is a contiguous range.
+int plan_tasks(const int32_t* row_ptr, int64_t m, int split_, int f, std::vector<cumf::Task>& t,
+               std::vector<cumf::HeavyRow>& hv, int64_t& nslots) {
+  const int64_t split = f <= kMaxFusedF ? std::max(split_, 1) : INT64_MAX;
+  t.clear();
+  hv.clear();
+  t.reserve((size_t)m);
+  int64_t slot = 0;
+  for (int64_t u = 0; u < m; ++u) {
+    const int64_t n = (int64_t)row_ptr[u + 1] - row_ptr[u];
+    if (n < 0) {
+      cumf::set_last_error("row pointers decrease at row " + std::to_string(u));
+      return CUMF_ERR_INVALID;
+    }
+    if (n <= split) {
+      t.push_back({(int32_t)u, row_ptr[u], (int32_t)n, -1});
+    } else {
+      const int64_t nc = (n + split - 1) / split;
+      hv.push_back({(int32_t)u, (int32_t)slot, (int32_t)nc, 0});
+      for (int64_t c = 0; c < nc; ++c)
+        t.push_back({(int32_t)u, (int32_t)(row_ptr[u] + c * split), (int32_t)std::min<int64_t>(split, n - c * split),
+                     (int32_t)(slot + c)});
+      slot += nc;
+    }
+  }
+  std::stable_sort(t.begin(), t.end(), [](const cumf::Task& a, const cumf::Task& b) { return a.count > b.count; });
+  nslots = slot;
+  return CUMF_OK;
+}
+
+// Wavefront task list of one rating matrix side on the device.
 struct Plan {
   int64_t rows = 0, nnz = 0, ntasks = 0, nheavy = 0, nslots = 0;
   int split = 0;
@@ -70,35 +104,13 @@ struct Plan {
   DevBuf tasks, heavy, partial;
 
   int build(const int32_t* row_ptr, int64_t m, int split_, int f) {
-    // Above kMaxFusedF the work-group-per-row kernel solves whole rows:
-    // nothing is split and no partial Grams are needed.
-    const int64_t split = f <= kMaxFusedF ? split_ : INT64_MAX;
     std::vector<cumf::Task> t;
     std::vector<cumf::HeavyRow> hv;
-    t.reserve((size_t)m);
-    int64_t slot = 0;
-    for (int64_t u = 0; u < m; ++u) {
-      const int64_t n = (int64_t)row_ptr[u + 1] - row_ptr[u];
-      if (n < 0) {
-        cumf::set_last_error("row pointers decrease at row " + std::to_string(u));
-        return CUMF_ERR_INVALID;
-      }
-      if (n <= split) {
-        t.push_back({(int32_t)u, row_ptr[u], (int32_t)n, -1});
-      } else {
-        const int64_t nc = (n + split - 1) / split;
-        hv.push_back({(int32_t)u, (int32_t)slot, (int32_t)nc, 0});
-        for (int64_t c = 0; c < nc; ++c)
-          t.push_back({(int32_t)u, (int32_t)(row_ptr[u] + c * split), (int32_t)std::min<int64_t>(split, n - c * split),
-                       (int32_t)(slot + c)});
-        slot += nc;
-      }
-    }
-    std::stable_sort(t.begin(), t.end(),
-                     [](const cumf::Task& a, const cumf::Task& b) { return a.count > b.count; });
+    int64_t slots = 0;
+    CUMF_TRY(plan_tasks(row_ptr, m, split_, f, t, hv, slots));
     rows = m;
     nnz = row_ptr[m];
-    this->split = f <= kMaxFusedF ? (int)split : 0;
+    split = f <= kMaxFusedF ? split_ : 0;
     ntasks = (int64_t)t.size();
     for (int b = 0; b < kBuckets; ++b) bucket_t0[b] = bucket_t1[b] = 0;
     for (int64_t i = 0; i < ntasks; ++i) {
@@ -108,7 +120,7 @@ struct Plan {
       bucket_t1[b] = i + 1;
     }
     nheavy = (int64_t)hv.size();
-    nslots = slot;
+    nslots = slots;
     CUMF_TRY(tasks.alloc(sizeof(cumf::Task) * t.size()));
     CUMF_TRY(heavy.alloc(sizeof(cumf::HeavyRow) * hv.size()));
     CUMF_TRY(partial.alloc(sizeof(float) * cumf::partial_floats_per_slot(f) * (size_t)nslots));
@@ -904,6 +916,29 @@ int cumf_reset_stats(cumf_handle* h) {
 hipStream_t cumf_get_stream(cumf_handle* h) { return h ? h->stream : nullptr; }
 
 int cumf_debug_gram_dim(int f) { return (f >= 1 && f <= kMaxFusedF) ? cumf::dump_dim(f) : -1; }
+
+int cumf_debug_plan(const int32_t* row_ptr, int64_t m, int split, int f, int32_t* tasks, int64_t task_cap,
+                    int64_t* ntasks, int32_t* heavy, int64_t heavy_cap, int64_t* nheavy) {
+  cumf::clear_last_error();
+  if (!row_ptr || m < 0 || !ntasks || !nheavy || check_f(f) != CUMF_OK || split < 1) {
+    cumf::set_last_error("cumf_debug_plan: bad arguments");
+    return CUMF_ERR_INVALID;
+  }
+  std::vector<cumf::Task> t;
+  std::vector<cumf::HeavyRow> hv;
+  int64_t slots = 0;
+  CUMF_TRY(plan_tasks(row_ptr, m, split, f, t, hv, slots));
+  *ntasks = (int64_t)t.size();
+  *nheavy = (int64_t)hv.size();
+  if ((int64_t)t.size() > task_cap || (int64_t)hv.size() > heavy_cap || (!t.empty() && !tasks) ||
+      (!hv.empty() && !heavy)) {
+    cumf::set_last_error("cumf_debug_plan: output arrays too small");
+    return CUMF_ERR_INVALID;
+  }
+  if (!t.empty()) std::memcpy(tasks, t.data(), sizeof(cumf::Task) * t.size());
+  if (!hv.empty()) std::memcpy(heavy, hv.data(), sizeof(cumf::HeavyRow) * hv.size());
+  return CUMF_OK;
+}
 
 int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t nrows, float* out) {
   cumf::clear_last_error();

@@ -250,6 +250,13 @@ int cumf_autotune(cumf_handle* h, int reps, const char* path);
  * block, b sits in column round_up(f,4).  out: host, nrows * D * D. */
 int cumf_debug_gram_dim(int f);
 int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t nrows, float* out);
+/* The host task planner alone (no device): the wavefront tasks of a CSR
+ * with the given split size, 4 int32 each (row, first rating, count, partial
+ * slot or -1), longest first, and the split rows, 4 int32 each (row, first
+ * slot, chunks, 0).  *ntasks / *nheavy receive the counts even when the
+ * capacities are too small (then CUMF_ERR_INVALID). */
+int cumf_debug_plan(const int32_t* row_ptr, int64_t m, int split, int f, int32_t* tasks, int64_t task_cap,
+                    int64_t* ntasks, int32_t* heavy, int64_t heavy_cap, int64_t* nheavy);
 /* Runs one v_mfma_f32_16x16x4_f32 with A[i][k] = 100 i + k, B[k][j] =
  * 1000 k + j and returns the 64 x 4 result registers (host out[256]). */
 int cumf_debug_mfma_layout(int device, float* out);

@@ -52,9 +52,15 @@ def test_data_lib_exports():
     assert _declared("cumf_data.h") <= exp
 
 
+def test_cpu_lib_exports():
+    exp = _exports(_paths.CPU_LIB)
+    assert _declared("cumf_cpu.h") <= exp
+
+
 def test_libraries_load_without_gpu():
     ctypes.CDLL(_paths.ALS_LIB)
     ctypes.CDLL(_paths.DATA_LIB)
+    ctypes.CDLL(_paths.CPU_LIB)
 
 
 def test_product_does_not_link_the_oracle():

@@ -10,7 +10,7 @@ from cumf import _paths
 from oracle import oracle as O
 from tests import _data
 
-LIB = C.CDLL(os.path.join(_paths.PKG_DIR, "libcumf_cpu.so"))
+LIB = C.CDLL(_paths.CPU_LIB)
 P = C.c_void_p
 LIB.cumf_cpu_half_sweep.argtypes = [C.c_int64, C.c_int64, C.c_int, P, P, P, P, C.c_float, P, C.c_int]
 LIB.cumf_cpu_half_sweep.restype = C.c_int
