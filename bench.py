@@ -5,24 +5,39 @@ plain FP32 path at equal held-out RMSE.
 Workload (default): the Netflix-shaped synthetic rating matrix
 m=480,189 users x n=17,770 items, 99,072,112 training ratings (power-law
 degrees; rows Zipf(1.0) capped at 17,000, columns Zipf(0.8)), a 1% held-out
-set, f=100, lambda=0.058.  One step = one ALS iteration = X half-sweep
-(users from items over the CSR) + Theta half-sweep (items from users over
-the CSC), with ratings and factors already resident in device memory.
+set, f=100, lambda=0.058, start init_factors(17770, 100, seed 0).  One step
+= one ALS iteration = X half-sweep (users from items over the CSR) + Theta
+half-sweep (items from users over the CSC), with ratings and factors
+resident in device memory.
 
-Default path: Gram on BF16 MFMA (FP32 accumulation), FP16 conjugate
-gradients with FP32 refinement and an FP64 residual, Cholesky fallback,
-each row warm-started from its current factor.  The plain FP32 path (FP32
-Gram + FP32 Cholesky) runs the same iterations from the same start in the
-same process for the comparison; --no-plain skips it.
+Default path (the combined form): Gram on BF16 MFMA with FP32
+accumulation; per degree bucket a launch shape and a solver (FP32 Cholesky,
+or FP16 conjugate gradients inside FP32 refinement with an FP64 residual
+and Cholesky fallback) chosen by the library's autotuner on this workload
+(untimed); every row solved to a relative residual of 1e-5 against the
+Gram the GPU formed; each half-sweep replayed from a captured hipGraph;
+CG started from the row's current factor.
 
-Prints ONE JSON line: the metric, throughput figures (s/iteration,
-ratings/s, hermitian TFLOP/s), the roofline of the dominant kernel
-(measured live with HIP events on the stream it runs on) and a CPU
-baseline (the FP64 checker timed on this host's cores on a row sample).
+Per path, in order:
+  1. training: 10 iterations from the seeded start, timed as a whole, and
+     the held-out RMSE after them against the FP64 oracle's 10-iteration
+     RMSE where a fixture holds it (tests/golden/netflix_f100_fp64.npz);
+  2. W untimed warm-up iterations, then K timed iterations (value);
+  3. per-kernel times of the same launches without graph replay (one
+     more iteration with HIP events) for the roofline, then one X and one
+     Theta half-sweep timed on their own.
+The plain FP32 path (FP32 Gram + FP32 Cholesky, the drop-in's arithmetic)
+runs the same steps from the same start for the comparison; the drop-in
+entry point doALSWithCSR (host factors in and out, PCIe included) is timed
+on one X and one Theta half-sweep.
+
+Prints ONE JSON line.
 """
 import argparse
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -37,16 +52,30 @@ METRIC = "ALS s/iteration & hermitian TFLOP/s at f=100, vs plain FP32 at equal R
 PEAK_FP32_MFMA_TFLOPS = 157.3   # MI355X dense FP32 matrix peak
 PEAK_BF16_MFMA_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 PEAK_HBM_GBS = 8000.0           # MI355X HBM3E
+FIXTURES = {("netflix_480k_17770_99m", 100, 0.058): os.path.join(REPO, "tests", "golden", "netflix_f100_fp64.npz")}
+TRAIN_ITERS = 10
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(ds, f, lam, theta, X, target_s):
-    """Times the FP64 checker's half-sweeps on row samples of both sides and
-    scales to one iteration by ratings (the per-rating cost of the normal
-    equations is what the sample measures)."""
+    """The FP64 normal-equation path (the oracle's C code, OpenMP over every
+    host core) timed on this host: one X half-sweep and one Theta
+    half-sweep, each on all rows if that fits in about target_s / 2 seconds,
+    else on an evenly strided row sample scaled by ratings."""
     from oracle import oracle as O
 
     threads = O.max_threads()
@@ -55,8 +84,6 @@ def cpu_baseline(ds, f, lam, theta, X, target_s):
     for side, (rp, ci, cv, src) in {"x": (ds.csr_row_ptr, ds.csr_col, ds.csr_val, theta),
                                      "theta": (ds.csc_col_ptr, ds.csc_row, ds.csc_val, X)}.items():
         rows = rp.shape[0] - 1
-        # Probe with a small stride sample, then size the real sample to
-        # about target_s / 2 seconds per side.
         stride = max(1, rows // 200)
         for attempt in range(3):
             sel = np.arange(0, rows, stride)
@@ -72,47 +99,75 @@ def cpu_baseline(ds, f, lam, theta, X, target_s):
                 break
             stride = max(1, int(stride * dt / (target_s / 2)))
         total += ds.nnz / rate
-        sample_desc.append(f"{side}: every {stride}th row ({sel.shape[0]} rows, {int(sub_rp[-1])} ratings, {dt:.1f}s)")
-    return {"value": total, "unit": "s/iteration", "cores": threads, "kind": "port",
-            "sample": "FP64 normal equations + Cholesky on " + "; ".join(sample_desc)
-                      + "; scaled to all 2 x 99,072,112 ratings of one iteration",
+        what = "all rows" if stride == 1 else f"every {stride}th row"
+        sample_desc.append(f"{side}: {what} ({sel.shape[0]} rows, {int(sub_rp[-1])} ratings, {dt:.1f}s)")
+    return {"value": total, "unit": "s/iteration", "cores": threads, "kind": "port", "cpu": cpu_model(),
+            "sample": "FP64 normal equations + Cholesky (the oracle's C restatement) on " + "; ".join(sample_desc)
+                      + "; scaled by ratings to one iteration of 2 x %d ratings" % ds.nnz,
             "ratings_per_s": ds.nnz / total}
 
 
-def measured_traffic(workload, precision, solver, f):
+def measured_traffic(workload, path, f):
     """HBM bytes per launch of the fused kernel for this workload and path,
     from the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE,
     each scaled by what the counter reads for a copy of known size); None if
     that configuration was not profiled."""
-    path = os.path.join(REPO, "profiles", "traffic_per_launch.txt")
+    fn = os.path.join(REPO, "profiles", "traffic_per_launch.txt")
     try:
-        with open(path) as fh:
+        with open(fn) as fh:
             for line in fh:
                 parts = line.split()
-                if len(parts) == 5 and not line.startswith("#") and parts[:4] == [workload, precision, solver, str(f)]:
-                    return float(parts[4])
+                if len(parts) == 4 and not line.startswith("#") and parts[:3] == [workload, path, str(f)]:
+                    return float(parts[3])
     except OSError:
         pass
     return None
 
 
-def precision_report(ds, f, lam, theta, X, nrows):
-    """Error of one X half-sweep against the FP64 oracle on sampled rows:
-    relative residual ||b - A x|| / ||b|| and factor error
-    ||x - x*|| / ||x*|| with A, b, x* formed in FP64 from the same theta."""
-    from oracle import oracle as O
-    rows = np.linspace(0, ds.m - 1, nrows).astype(np.int64)
-    res, err = [], []
-    for u in rows:
-        if ds.csr_row_ptr[u + 1] == ds.csr_row_ptr[u]:
-            continue
-        A, b = O.row_system(ds.csr_row_ptr, ds.csr_col, ds.csr_val, theta, f, lam, int(u))
-        x = X[u].astype(np.float64)
-        xs = np.linalg.solve(A, b)
-        res.append(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
-        err.append(np.linalg.norm(x - xs) / np.linalg.norm(xs))
+def _bf16_round(a):
+    """FP32 -> BF16 (nearest even) -> FP32: the factor copy a BF16 Gram reads."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32).view(np.float32)
+
+
+def precision_report(ds, f, lam, theta, X, nrows, bf16):
+    """Error of one X half-sweep on sampled rows (the heaviest and lightest
+    eighths and a seeded uniform draw), against the FP64 normal equations
+    formed here with numpy: A = Y^T Y + lambda n_u I, b = Y^T r from the same
+    factors -- as the GPU read them (BF16-rounded for a BF16 Gram) and as
+    FP32.  Relative residual ||b - A x|| / ||b|| and factor error
+    ||x - x*|| / ||x*||, x* = A^-1 b in FP64."""
+    rp = ds.csr_row_ptr
+    deg = np.diff(rp)
+    live = np.flatnonzero(deg > 0)
+    order = live[np.argsort(-deg[live], kind="stable")]
+    pick = list(order[:nrows // 8]) + list(order[-(nrows // 8):])
+    rest = np.setdiff1d(live, pick)
+    pick += list(np.random.default_rng(11).choice(rest, nrows - len(pick), replace=False))
+    rows = np.sort(np.array(pick, np.int64))
+
+    def errors(src):
+        res, err = [], []
+        for u in rows:
+            s, e = rp[u], rp[u + 1]
+            Y = src[ds.csr_col[s:e]].astype(np.float64)
+            r = ds.csr_val[s:e].astype(np.float64)
+            A = Y.T @ Y + np.float64(np.float32(lam) * np.float32(e - s)) * np.eye(f)
+            b = Y.T @ r
+            x = X[u].astype(np.float64)
+            xs = np.linalg.solve(A, b)
+            res.append(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+            err.append(np.linalg.norm(x - xs) / np.linalg.norm(xs))
+        return res, err
+
     q = lambda v: {"median": float(np.median(v)), "p99": float(np.percentile(v, 99)), "max": float(np.max(v))}
-    return {"rows": len(res), "relative_residual": q(res), "factor_rel_error": q(err)}
+    out = {"rows": int(rows.shape[0]), "sample": "heaviest and lightest eighths + seeded uniform draw"}
+    if bf16:
+        res, err = errors(_bf16_round(theta))
+        out["vs_system_on_bf16_factors"] = {"relative_residual": q(res), "factor_rel_error": q(err)}
+    res, err = errors(theta)
+    out["vs_system_on_fp32_factors"] = {"relative_residual": q(res), "factor_rel_error": q(err)}
+    return out
 
 
 def train_network(G, ds, f, lam, args, prec):
@@ -121,7 +176,7 @@ def train_network(G, ds, f, lam, args, prec):
     import cumf
     from cumf import mlp as NN
     net = NN.MLP(f, seed=1)
-    hh = G.ALS(f, lam, device=0, split=args.split, precision=prec)
+    hh = G.ALS(f, lam, device=0, precision=prec)
     hh.set_dataset(ds)
     hh.set_factors(theta=cumf.init_factors(ds.n, f, 1))
     t1 = time.perf_counter()
@@ -137,26 +192,34 @@ def train_network(G, ds, f, lam, args, prec):
     return net, learned
 
 
-def run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net=None, tune=True):
-    """W untimed + K timed iterations of one path; returns its figures."""
-    h = G.ALS(f, lam, device=0, timing=1, split=args.split, precision=prec, use_graph=int(args.graph),
-              solver=solver, cg_iters=args.cg_iters, cg_rounds=args.cg_rounds, cg_tol=args.cg_tol,
-              warm_start=warm)
+def run_path(G, ds, theta0, f, lam, args, name, opts, tune, net=None, report_rows=0):
+    """Training, steady-state timing and per-kernel split of one path."""
+    h = G.ALS(f, lam, device=0, **opts)
     if net is not None:
         h.set_mlp(net)
     h.set_dataset(ds)
     h.set_factors(theta=theta0)
     tuning = None
-    if tune and args.tile_table:
-        h.load_tile_table(args.tile_table)
-        tuning = {"tile_table": args.tile_table}
-    elif tune and args.autotune:
+    if tune == "autotune":
         out = os.path.join(REPO, "gpurun_out", "tile_table_%s_f%d.txt" % (args.workload, f))
         os.makedirs(os.path.dirname(out), exist_ok=True)
         t1 = time.perf_counter()
         h.autotune(reps=2, path=out)
-        tuning = {"autotune_s": time.perf_counter() - t1, "table": open(out).read().splitlines()}
-        log(f"autotuned in {tuning['autotune_s']:.1f}s -> {out}")
+        tuning = {"autotune_s": time.perf_counter() - t1, "table": [l for l in open(out).read().splitlines()
+                                                                    if not l.startswith("#")]}
+        log(f"{name}: autotuned in {tuning['autotune_s']:.1f}s -> {out}")
+    elif tune:
+        h.load_tile_table(tune)
+        tuning = {"tile_table": tune}
+    # 1. training from the seeded start
+    h.synchronize()
+    t0 = time.perf_counter()
+    h.iterate(TRAIN_ITERS)
+    h.synchronize()
+    train_s = time.perf_counter() - t0
+    train_rmse = h.rmse()
+    train_fallback = h.stats()["cg_fallback_rows"]
+    # 2. steady state: W warm-up + K timed iterations
     for _ in range(args.warmup):
         h.iterate(1)
     h.synchronize()
@@ -166,53 +229,102 @@ def run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net=None, tune=Tru
     h.synchronize()
     elapsed = time.perf_counter() - t0
     st = h.stats()
-    rmse = h.rmse()
     X, T = h.get_factors()
-    # Per-half-sweep split, after the timed region (one more iteration).
-    sweep_ms = {}
-    X_next = None
-    for name, fn in (("x", h.update_x), ("theta", h.update_theta)):
-        h.synchronize()
+    h.close()
+    # 3. per-kernel times without graph replay, and the half-sweep split
+    eo = dict(opts, use_graph=0, timing=1)
+    he = G.ALS(f, lam, device=0, **eo)
+    if net is not None:
+        he.set_mlp(net)
+    he.set_dataset(ds)
+    if tune == "autotune":
+        he.load_tile_table(os.path.join(REPO, "gpurun_out", "tile_table_%s_f%d.txt" % (args.workload, f)))
+    elif tune:
+        he.load_tile_table(tune)
+    he.set_factors(X=X, theta=T)
+    he.iterate(1)
+    he.synchronize()
+    he.reset_stats()
+    he.iterate(1)
+    he.synchronize()
+    kst = he.stats()
+    sweep_ms, report = {}, None
+    for side, fn in (("x", he.update_x), ("theta", he.update_theta)):
+        he.synchronize()
         t1 = time.perf_counter()
         fn()
-        h.synchronize()
-        sweep_ms[name] = (time.perf_counter() - t1) * 1e3
-        if name == "x" and args.precision_report:
-            X_next = h.get_factors()[0]
-    opt = h.opt
-    h.close()
-    report = precision_report(ds, f, lam, T, X_next, args.precision_report) if X_next is not None else None
-    return {"s": elapsed / args.steps, "rmse": rmse, "stats": st, "sweep_ms": sweep_ms, "X": X, "T": T,
-            "opt": opt, "tuning": tuning, "precision_report": report}
+        he.synchronize()
+        sweep_ms[side] = (time.perf_counter() - t1) * 1e3
+        if side == "x" and report_rows:
+            Xn, Tn = he.get_factors()
+            report = precision_report(ds, f, lam, Tn, Xn, report_rows, opts.get("precision", 0) != 0)
+    opt = he.opt
+    he.close()
+    res = {"s": elapsed / args.steps, "stats": st, "kernel_stats": kst, "sweep_ms": sweep_ms, "X": X, "T": T,
+           "opt": opt, "tuning": tuning, "precision_report": report, "train_s": train_s,
+           "train_rmse": train_rmse, "train_fallback": train_fallback, "rmse": None}
+    log(f"{name}: train {TRAIN_ITERS} it {train_s:.3f}s rmse {train_rmse:.6f}; steady {res['s'] * 1e3:.3f} ms/it; "
+        f"half-sweeps {sweep_ms}")
+    return res
+
+
+def time_dropin(G, ds, f, lam, theta, X):
+    """doALSWithCSR (host factors in and out) on one X and one Theta
+    half-sweep, the second call of each timed (the first fills the per-device
+    plan cache); the caller's CSR upload into mallocBest memory is timed on
+    its own."""
+    out = {}
+    for side, (rp, ci, cv, src, rows, other) in {
+            "x": (ds.csr_row_ptr, ds.csr_col, ds.csr_val, theta, ds.m, ds.n),
+            "theta": (ds.csc_col_ptr, ds.csc_row, ds.csc_val, X, ds.n, ds.m)}.items():
+        t_call = t_up = 0.0
+        for rep in range(2):
+            t0 = time.perf_counter()
+            bufs = [G.DeviceBuffer.from_array(a) for a in (rp, ci, cv)]
+            t1 = time.perf_counter()
+            Y = np.zeros((rows, f), np.float32)
+            G.lib().doALSWithCSR(None, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, G._ptr(np.ascontiguousarray(src)),
+                                 G._ptr(Y), rows, other, f, int(ci.shape[0]), lam, 1)
+            t2 = time.perf_counter()
+            for b in bufs:
+                b.ptr = G.P()
+            err = G.last_error()
+            if err:
+                raise G.CumfError(f"doALSWithCSR: {err}")
+            t_up, t_call = t1 - t0, t2 - t1
+        out[side] = {"call_ms": t_call * 1e3, "csr_upload_ms": t_up * 1e3}
+    out["ms_per_iteration"] = out["x"]["call_ms"] + out["theta"]["call_ms"]
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="netflix_480k_17770_99m")
     ap.add_argument("--f", type=int, default=100)
     ap.add_argument("--lam", type=float, default=0.058)
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "bf16gram"], default="bf16gram")
-    ap.add_argument("--solver", choices=["cholesky", "cg"], default="cg")
+    ap.add_argument("--solver", choices=["cholesky", "cg"], default="cg",
+                    help="the handle's solver; with cg the autotuner may pick Cholesky per degree bucket")
     ap.add_argument("--cg-iters", type=int, default=0)
     ap.add_argument("--cg-rounds", type=int, default=0)
-    ap.add_argument("--cg-tol", type=float, default=0.0)
+    ap.add_argument("--cg-tol", type=float, default=0.0, help="0: the library default (1e-5)")
     ap.add_argument("--warm", choices=["prev", "zero", "mlp", "best"], default="prev",
                     help="CG start; mlp/best train the warm-start network first (untimed)")
-    ap.add_argument("--mlp-harvest", type=int, default=3, help="plain iterations harvested for training")
-    ap.add_argument("--mlp-epochs", type=int, default=4)
+    ap.add_argument("--mlp-harvest", type=int, default=5, help="plain iterations harvested for training")
+    ap.add_argument("--mlp-epochs", type=int, default=2)
     ap.add_argument("--mlp-batch", type=int, default=4096)
-    ap.add_argument("--graph", action="store_true", help="replay each half-sweep as a captured hipGraph")
-    ap.add_argument("--autotune", action="store_true",
-                    help="tune the degree-bucket launch shapes on this workload first (untimed)")
+    ap.add_argument("--no-graph", action="store_true", help="launch each half-sweep eagerly")
+    ap.add_argument("--no-tune", action="store_true", help="one launch per kernel, no tile table")
     ap.add_argument("--tile-table", default="", help="load degree-bucket launch shapes from this file")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
-    ap.add_argument("--precision-report", type=int, default=0, metavar="ROWS",
-                    help="per-row FP64 residual and factor error of one more X half-sweep on ROWS sampled rows")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-dropin", action="store_true", help="skip timing doALSWithCSR")
+    ap.add_argument("--precision-report", type=int, default=256, metavar="ROWS",
+                    help="FP64 residual and factor error of one X half-sweep on ROWS sampled rows (0: off)")
+    ap.add_argument("--cpu-seconds", type=float, default=24.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES bytes (HBM counter calibration)")
@@ -237,6 +349,8 @@ def main():
         f"in {time.time() - t0:.1f}s")
     f, lam = args.f, args.lam
     theta0 = cumf.init_factors(ds.n, f, 0)
+    fx_path = FIXTURES.get((args.workload, f, round(lam, 6)))
+    fp64_rmse = float(np.load(fx_path)["rmse"][TRAIN_ITERS - 1]) if fx_path and os.path.exists(fx_path) else None
     prec = {"fp32": G.CUMF_PREC_FP32, "bf16gram": G.CUMF_PREC_BF16_GRAM}[args.precision]
     solver = {"cholesky": G.CUMF_SOLVER_CHOLESKY, "cg": G.CUMF_SOLVER_CG}[args.solver]
     warm = {"prev": G.CUMF_WARM_PREV, "zero": G.CUMF_WARM_ZERO, "mlp": G.CUMF_WARM_MLP,
@@ -246,45 +360,47 @@ def main():
     learned, net = None, None
     if warm in (G.CUMF_WARM_MLP, G.CUMF_WARM_BEST):
         net, learned = train_network(G, ds, f, lam, args, prec)
-
-    res = run_path(G, ds, theta0, f, lam, args, prec, solver, warm, net)
-    st, opt = res["stats"], res["opt"]
-    log(f"timed {args.steps} iterations: {res['s']:.4f} s/iteration; rmse={res['rmse']:.5f}; stats={st}")
+    opts = dict(precision=prec, solver=solver, split=args.split, cg_iters=args.cg_iters, cg_rounds=args.cg_rounds,
+                cg_tol=args.cg_tol, warm_start=warm, use_graph=0 if args.no_graph else 1, timing=0)
+    tune = None if args.no_tune else (args.tile_table or "autotune")
+    res = run_path(G, ds, theta0, f, lam, args, "path", opts, tune, net, args.precision_report)
+    st, kst, opt = res["stats"], res["kernel_stats"], res["opt"]
+    if learned is not None:
+        learned["cg_fallback_rows_training"] = res["train_fallback"]
     is_plain = prec == G.CUMF_PREC_FP32 and solver == G.CUMF_SOLVER_CHOLESKY
     plain = None
     if not is_plain and not args.no_plain:
-        plain = run_path(G, ds, theta0, f, lam, args, G.CUMF_PREC_FP32, G.CUMF_SOLVER_CHOLESKY,
-                         G.CUMF_WARM_PREV, tune=False)
-        log(f"plain FP32: {plain['s']:.4f} s/iteration; rmse={plain['rmse']:.5f}")
+        popts = dict(precision=G.CUMF_PREC_FP32, solver=G.CUMF_SOLVER_CHOLESKY, split=args.split,
+                     use_graph=0 if args.no_graph else 1, timing=0)
+        plain = run_path(G, ds, theta0, f, lam, args, "plain fp32", popts, None, None, args.precision_report)
 
     s_per_iter = res["s"]
     # Unique symmetric Gram work: f(f+1)/2 entries, one multiply-add each, per
     # rating and half-sweep.
     gram_flops_iter = 2.0 * ds.nnz * f * (f + 1)
-    # One "launch" = all als_task_kernel launches of one half-sweep (one, or
-    # one per degree bucket with a tile table).
-    if st["task_launches"]:  # per-kernel events
-        kern_ms_total, launches, kname = st["task_ms"], st["sweeps"], "als_task_kernel (fused gram+solve)"
-    else:  # graph replays: whole half-sweeps
-        kern_ms_total, launches, kname = st["iter_ms"], st["iters"], "one captured half-sweep (hipGraph)"
-    task_ms = kern_ms_total / max(launches, 1)
-    # all kernels of the sweeps (fused Gram + solve, split-row finish, fallback)
-    sweep_kernel_ms = (st["task_ms"] + st["finish_ms"] + st["fallback_ms"]) if st["task_launches"] else kern_ms_total
-    traffic = measured_traffic(args.workload, args.precision, args.solver, f) if st["task_launches"] else None
-    gram_flops_launch = st["gram_flops"] / max(launches, 1)
+    # The dominant kernel: als_task_kernel, all its launches of one half-sweep
+    # (one, or one per degree bucket) = one "launch"; per-kernel HIP events of
+    # the eager iteration after the timed region (graph replays carry none).
+    launches = max(kst["sweeps"], 1)
+    task_ms = kst["task_ms"] / launches
+    sweep_kernel_ms = (kst["task_ms"] + kst["finish_ms"] + kst["fallback_ms"]) / launches * 2
+    dflt = G.default_options()  # 0 in an option means the library default
+    iters, rounds = opt.cg_iters or dflt.cg_iters, opt.cg_rounds or dflt.cg_rounds
+    tol = opt.cg_tol if opt.cg_tol > 0 else dflt.cg_tol
+    pname = "bf16" if prec != G.CUMF_PREC_FP32 else "fp32"
+    traffic_key = f"{pname}-{args.solver}" + ("" if args.no_tune else "-tuned")
+    traffic = measured_traffic(args.workload, traffic_key, f)
+    gram_flops_launch = kst["gram_flops"] / launches
     achieved = gram_flops_launch / (task_ms * 1e-3) / 1e12
     peak = PEAK_FP32_MFMA_TFLOPS if prec == G.CUMF_PREC_FP32 else PEAK_BF16_MFMA_TFLOPS
     factor_bytes = 4 * f if prec == G.CUMF_PREC_FP32 else 2 * ((f + 7) // 8 * 8)
     gather_bytes_launch = ds.nnz * (8 + factor_bytes)  # col + rating + one factor row per rating
     if solver == G.CUMF_SOLVER_CHOLESKY:
-        path = ("fp32" if prec == G.CUMF_PREC_FP32 else "bf16") + " gram (MFMA) + fp32 cholesky"
+        path = f"{pname} gram (MFMA, fp32 accumulate) + fp32 cholesky (MFMA panel and trailing updates)"
     else:
-        dflt = G.default_options()  # 0 in an option means the library default
-        iters, rounds = opt.cg_iters or dflt.cg_iters, opt.cg_rounds or dflt.cg_rounds
-        tol = opt.cg_tol if opt.cg_tol > 0 else dflt.cg_tol
-        path = (("fp32" if prec == G.CUMF_PREC_FP32 else "bf16") + " gram (MFMA, fp32 accumulate) + "
-                f"fp16 cg ({iters} steps x up to {rounds} fp32 refinement rounds, fp64 acceptance residual, "
-                f"tol {tol:g}, cholesky fallback, start {args.warm})")
+        path = (f"{pname} gram (MFMA, fp32 accumulate) + per degree bucket fp32 cholesky or fp16 cg "
+                f"({iters} steps x up to {rounds} fp32 refinement rounds, fp64 residual, start {args.warm}) "
+                "as tuned; every row to relative residual <= %g against the GPU's Gram" % tol)
     rec = {
         "metric": METRIC,
         "value": s_per_iter,
@@ -296,26 +412,31 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp32" if prec == G.CUMF_PREC_FP32 else "bf16 gram")
-                 + (" / fp32 solve" if solver == G.CUMF_SOLVER_CHOLESKY
-                    else " / fp16 cg + fp32 refinement + fp64 residual"),
-        "data": "synthetic",
+        "dtype": f"{'bf16' if prec else 'fp32'} gram / fp32 accumulate"
+                 + (" / fp32 cholesky" if solver == G.CUMF_SOLVER_CHOLESKY
+                    else " / fp32 cholesky or fp16 cg + fp32 refinement + fp64 residual"),
+        "data": "synthetic (seeded generator: power-law degrees, rank-20 latent ratings 1-5); "
+                "factors from init_factors seed 0",
         "config": {"workload": args.workload, "m": ds.m, "n": ds.n, "nnz": ds.nnz, "f": f,
-                   "lambda": lam, "path": path, "graph": bool(args.graph), "split": args.split or 16384,
-                   "iterations_timed": f"{args.warmup + 1}..{args.warmup + args.steps}"},
+                   "lambda": lam, "path": path, "graph": not args.no_graph, "split": opt.split or dflt.split,
+                   "tol": tol if solver == G.CUMF_SOLVER_CG else None,
+                   "iterations_timed": f"{TRAIN_ITERS + args.warmup + 1}..{TRAIN_ITERS + args.warmup + args.steps}"},
         "ratings_per_s": ds.nnz / s_per_iter,
-        "hermitian_tflops": gram_flops_iter / (sweep_kernel_ms / args.steps * 1e-3) / 1e12,
-        "heldout_rmse": res["rmse"],
+        "hermitian_tflops": gram_flops_iter / (sweep_kernel_ms * 1e-3) / 1e12,
+        "training": {"iterations": TRAIN_ITERS, "s": res["train_s"], "heldout_rmse": res["train_rmse"],
+                     "fp64_oracle_rmse": fp64_rmse,
+                     "rmse_delta_vs_fp64_oracle": (res["train_rmse"] - fp64_rmse) if fp64_rmse else None,
+                     "cg_fallback_rows": res["train_fallback"]},
         "failed_rows": st["failed_rows"],
         "cg_fallback_rows_per_iter": st["cg_fallback_rows"] / args.steps,
-        "kernel_ms_per_iter": {"als_task_kernel": st["task_ms"] / args.steps,
-                               "als_finish_kernel": st["finish_ms"] / args.steps,
-                               "als_fallback_kernel": st["fallback_ms"] / args.steps,
-                               "warm_start": st["warm_ms"] / args.steps},
+        "kernel_ms_per_iter": {"als_task_kernel": kst["task_ms"],
+                               "als_finish_kernel": kst["finish_ms"],
+                               "als_fallback_kernel": kst["fallback_ms"],
+                               "warm_start": kst["warm_ms"]},
         "half_sweep_ms": res["sweep_ms"],
         "learned": learned,
         "batching": res["tuning"],
-        "roofline": {"kernel": kname, "bound": "mfma",
+        "roofline": {"kernel": "als_task_kernel (fused gram+solve), per half-sweep", "bound": "mfma",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "avg_launch_ms": task_ms,
@@ -328,12 +449,22 @@ def main():
     if res["precision_report"] is not None:
         rec["precision_report"] = res["precision_report"]
     if plain is not None:
-        rec["plain_fp32"] = {"s_per_iteration": plain["s"], "heldout_rmse": plain["rmse"],
-                             "half_sweep_ms": plain["sweep_ms"]}
+        rec["plain_fp32"] = {"s_per_iteration": plain["s"], "half_sweep_ms": plain["sweep_ms"],
+                             "training": {"iterations": TRAIN_ITERS, "s": plain["train_s"],
+                                          "heldout_rmse": plain["train_rmse"],
+                                          "rmse_delta_vs_fp64_oracle":
+                                              (plain["train_rmse"] - fp64_rmse) if fp64_rmse else None},
+                             "als_task_kernel_ms_per_iter": plain["kernel_stats"]["task_ms"]}
         if plain["precision_report"] is not None:
             rec["plain_fp32"]["precision_report"] = plain["precision_report"]
         rec["speedup_vs_plain_fp32"] = plain["s"] / s_per_iter
-        rec["rmse_delta_vs_plain_fp32"] = res["rmse"] - plain["rmse"]
+        rec["training_speedup_vs_plain_fp32"] = plain["train_s"] / res["train_s"]
+        rec["rmse_delta_vs_plain_fp32"] = res["train_rmse"] - plain["train_rmse"]
+    if not args.no_dropin:
+        log("timing doALSWithCSR")
+        rec["dropin_doALSWithCSR"] = time_dropin(G, ds, f, lam, theta0, res["X"])
+        rec["dropin_doALSWithCSR"]["note"] = ("legacy C ABI, FP32 Cholesky, host factors in and out "
+                                              "(PCIe included); not the value")
     if not args.no_cpu_baseline:
         log("timing CPU baseline")
         rec["cpu_baseline"] = cpu_baseline(ds, f, lam, theta0, res["X"], args.cpu_seconds)
