@@ -195,6 +195,8 @@ struct cumf_handle {
   // X side's rows.
   cumf_mlp* mlp = nullptr;
   DevBuf feat, x0;
+  int64_t x_sweeps = 0;           // X half-sweeps since the factors were last set
+  bool graph_net[2] = {false, false};  // the captured half-sweep runs the network
   // Rows the CG pass hands to the Cholesky fallback, and their count.
   DevBuf fb_list, fb_count;
   // Host copies of the row / column pointers (plans are rebuilt when the
@@ -331,6 +333,12 @@ bool uses_network(const cumf_handle* h) {
          (h->opt.warm_start == CUMF_WARM_MLP || h->opt.warm_start == CUMF_WARM_BEST);
 }
 
+// The network starts this X half-sweep (warm_sweeps limits it to the first
+// ones after the factors were set).
+bool network_now(const cumf_handle* h) {
+  return uses_network(h) && (h->opt.warm_sweeps <= 0 || h->x_sweeps < h->opt.warm_sweeps);
+}
+
 // Buffers of the learned warm start (allocated outside any capture).
 int prepare_network(cumf_handle* h) {
   if (!uses_network(h)) return CUMF_OK;
@@ -418,8 +426,9 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
   }
   const bool cg = h->opt.solver == CUMF_SOLVER_CG;
   a.warm = h->opt.warm_start;
-  if (side == 1 && a.warm >= CUMF_WARM_MLP) a.warm = CUMF_WARM_PREV;  // the network predicts X rows
-  if (side == 0 && uses_network(h)) {
+  // the network predicts X rows, and only while network_now() holds
+  if (a.warm >= CUMF_WARM_MLP && (side == 1 || !network_now(h))) a.warm = CUMF_WARM_PREV;
+  if (side == 0 && network_now(h)) {
     CUMF_TRY(prepare_network(h));
     const int K0 = cumf::mlp_k0(h->mlp);
     CUMF_TRY(timed(h, Timer::WARM, [&] {
@@ -522,6 +531,10 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
     cumf::set_last_error("f > " + std::to_string(kMaxFusedF) +
                          " runs the FP32 Gram + FP32 Cholesky path only");
     return CUMF_ERR_UNSUPPORTED;
+  }
+  if (o.warm_sweeps < 0) {
+    cumf::set_last_error("warm_sweeps must be >= 0");
+    return CUMF_ERR_INVALID;
   }
   if (o.cg_iters > 64 || o.cg_rounds > 16) {
     cumf::set_last_error("cg_iters must be <= 64 and cg_rounds <= 16");
@@ -725,6 +738,7 @@ int cumf_set_factors(cumf_handle* h, const float* X, const float* theta) {
   if (X) CUMF_TRY(upload_factors(h, h->X, h->Xb, X, h->m));
   if (theta) CUMF_TRY(upload_factors(h, h->T, h->Tb, theta, h->n));
   CUMF_HIP_TRY(hipStreamSynchronize(h->stream));
+  h->x_sweeps = 0;  // a new start: the transient sweeps begin again
   return CUMF_OK;
 }
 
@@ -815,15 +829,24 @@ int capture_half_sweep(cumf_handle* h, int side) {
 // One half-sweep: replayed from its graph with use_graph, else launched.
 int half_sweep(cumf_handle* h, int side) {
   if (h->opt.use_graph) {
-    if (!h->graph[side]) CUMF_TRY(capture_half_sweep(h, side));
+    // a captured X half-sweep that runs the network (or not) is recaptured
+    // when network_now() changes
+    const bool net = side == 0 && network_now(h);
+    if (!h->graph[side] || h->graph_net[side] != net) {
+      CUMF_TRY(capture_half_sweep(h, side));
+      h->graph_net[side] = net;
+    }
     CUMF_TRY(timed(h, Timer::ITER, [&] { return hipGraphLaunch(h->graph[side], h->stream); }));
     h->gram_flops += (side == 0 ? h->plan_x.nnz : h->plan_t.nnz) * (int64_t)h->f * (h->f + 1);
+    if (side == 0) ++h->x_sweeps;
     return CUMF_OK;
   }
-  return side == 0 ? run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(), h->X.as<float>(),
-                                    h->Tb, h->Xb)
-                   : run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(), h->T.as<float>(),
-                                    h->Xb, h->Tb);
+  const int rc = side == 0 ? run_half_sweep(h, 0, h->plan_x, h->rp, h->ci, h->rv, h->T.as<float>(),
+                                            h->X.as<float>(), h->Tb, h->Xb)
+                           : run_half_sweep(h, 1, h->plan_t, h->cp, h->ri, h->cv, h->X.as<float>(),
+                                            h->T.as<float>(), h->Xb, h->Tb);
+  if (rc == CUMF_OK && side == 0) ++h->x_sweeps;
+  return rc;
 }
 
 }  // namespace
@@ -1154,8 +1177,8 @@ int autotune_side(cumf_handle* h, int side, int reps, const TuneStart& start) {
   }
   // 4. Buckets launched one after another drain the GPU between launches:
   //    keep the per-bucket shapes only if a whole half-sweep with them beats
-  //    one launch of every task with the default shape and the handle's
-  //    solver.
+  //    one launch of every task with the default shape -- with the handle's
+  //    solver, or on a CG handle with the Cholesky solver for the whole side.
   {
     BucketCfg tuned[kBuckets];
     std::copy(h->tiles[side], h->tiles[side] + kBuckets, tuned);
@@ -1165,11 +1188,18 @@ int autotune_side(cumf_handle* h, int side, int reps, const TuneStart& start) {
       SideBufs sb = side_bufs(h, side);
       return run_half_sweep(h, side, *sb.p, *sb.rowp, *sb.col, *sb.val, sb.theta, sb.x, *sb.theta_b, *sb.x_b);
     };
-    double t_buckets = 0, t_single = 0;
-    CUMF_TRY(time_min_ms(h, reps, restore, sweep, &t_buckets));
-    for (int k = 0; k < kBuckets; ++k) h->tiles[side][k] = BucketCfg{dv, 0, -1};
-    CUMF_TRY(time_min_ms(h, reps, restore, sweep, &t_single));
-    if (t_buckets < t_single) std::copy(tuned, tuned + kBuckets, h->tiles[side]);
+    double t_best = 0;
+    CUMF_TRY(time_min_ms(h, reps, restore, sweep, &t_best));
+    for (int sol : solvers) {
+      for (int k = 0; k < kBuckets; ++k) h->tiles[side][k] = BucketCfg{dv, 0, sol};
+      double t = 0;
+      CUMF_TRY(time_min_ms(h, reps, restore, sweep, &t));
+      if (t < t_best) {
+        t_best = t;
+        for (int k = 0; k < kBuckets; ++k) tuned[k] = BucketCfg{dv, 0, sol};
+      }
+    }
+    std::copy(tuned, tuned + kBuckets, h->tiles[side]);
     h->tiled = was_tiled;
   }
   return CUMF_OK;

@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "als_kernels.h"
 
 namespace cumf {
@@ -299,13 +301,15 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
 //
 // Right-looking over the 16-row blocks K of the augmented Gram:
 //   1. chol_diag: the diagonal tile T is factored by Gauss elimination of
-//      [T | I] run column-wise -- lane c < 16 holds column c of T, lane
-//      16 + c column c of the identity, 16 registers each.  Pivot k scales
-//      row k by 1/sqrt(t_kk) and subtracts u_ki * row k from every row
-//      i > k, the multiplier u_ki read from lane i (v_readlane).  The T half
-//      ends as U_KK (U_KK^T U_KK = T), the identity half as
-//      W_K = U_KK^{-T}.  Pivots at or past f are not taken: those rows of
-//      U and W are zero, so they contribute nothing below.
+//      [T | I] run column-wise -- lane c of the even 16-lane rows holds
+//      column c of T, lane c of the odd rows column c of the identity, 16
+//      registers each.  Pivot k scales row k by 1/sqrt(t_kk) and subtracts
+//      u_ki * row k from every row i > k; the multipliers u_ki (lane i of
+//      the T row) are copied into the identity rows with one permlane swap
+//      and read by v_fmac_f32_dpp row_newbcast:i, one instruction per
+//      multiplier.  The T half ends as U_KK (U_KK^T U_KK = T), the identity
+//      half as W_K = U_KK^{-T}.  Pivots at or past f are not taken: those
+//      rows of U and W are zero, so they contribute nothing below.
 //   2. chol_panel: U_KJ = W_K A_KJ for J > K on v_mfma_f32_16x16x4f32.  In
 //      the accumulator layout (lane (g, c), register s holds M[4g+s][c]) a
 //      tile is the B operand of the four k-steps as it stands, with
@@ -320,6 +324,252 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
 // LDS (floats): W_K for every block (NB x 256), the diagonal tile being
 // factored (256), z and the second right-hand side's z (16 NB each).
 // ---------------------------------------------------------------------------
+// Elimination step of pivot K in the column-per-lane factorisation:
+// a[i] -= u_i * a[K] for i = K+1..15, where the multiplier u_i is the value
+// of u held by lane i of the lane's own 16-lane row, read (negated) with
+// DPP row_newbcast:i -- one v_fmac_f32_dpp per multiplier.  The s_nop
+// covers the two wait states a DPP read needs after the VALU write of u.
+template <int K>
+__device__ __forceinline__ void elim_dpp(float (&a)[16], float u) {
+  if constexpr (K == 0) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%15, %16 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%15, %16 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%15, %16 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%15, %16 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%15, %16 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%15, %16 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %6, -%15, %16 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %7, -%15, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %8, -%15, %16 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %9, -%15, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %10, -%15, %16 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %11, -%15, %16 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %12, -%15, %16 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %13, -%15, %16 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %14, -%15, %16 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[0]));
+  }
+  else if constexpr (K == 1) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%14, %15 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%14, %15 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%14, %15 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%14, %15 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%14, %15 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%14, %15 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %6, -%14, %15 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %7, -%14, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %8, -%14, %15 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %9, -%14, %15 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %10, -%14, %15 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %11, -%14, %15 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %12, -%14, %15 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %13, -%14, %15 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[1]));
+  }
+  else if constexpr (K == 2) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%13, %14 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%13, %14 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%13, %14 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%13, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%13, %14 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%13, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %6, -%13, %14 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %7, -%13, %14 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %8, -%13, %14 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %9, -%13, %14 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %10, -%13, %14 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %11, -%13, %14 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %12, -%13, %14 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[2]));
+  }
+  else if constexpr (K == 3) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%12, %13 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%12, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%12, %13 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%12, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%12, %13 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%12, %13 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %6, -%12, %13 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %7, -%12, %13 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %8, -%12, %13 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %9, -%12, %13 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %10, -%12, %13 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %11, -%12, %13 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[3]));
+  }
+  else if constexpr (K == 4) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%11, %12 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%11, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%11, %12 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%11, %12 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%11, %12 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%11, %12 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %6, -%11, %12 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %7, -%11, %12 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %8, -%11, %12 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %9, -%11, %12 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %10, -%11, %12 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[4]));
+  }
+  else if constexpr (K == 5) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%10, %11 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%10, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%10, %11 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%10, %11 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%10, %11 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%10, %11 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %6, -%10, %11 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %7, -%10, %11 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %8, -%10, %11 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %9, -%10, %11 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[6]), "+v"(a[7]), "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[5]));
+  }
+  else if constexpr (K == 6) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%9, %10 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%9, %10 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%9, %10 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%9, %10 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%9, %10 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%9, %10 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %6, -%9, %10 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %7, -%9, %10 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %8, -%9, %10 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[7]), "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[6]));
+  }
+  else if constexpr (K == 7) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%8, %9 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%8, %9 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%8, %9 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%8, %9 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%8, %9 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%8, %9 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %6, -%8, %9 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %7, -%8, %9 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[7]));
+  }
+  else if constexpr (K == 8) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%7, %8 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%7, %8 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%7, %8 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%7, %8 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%7, %8 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%7, %8 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %6, -%7, %8 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[8]));
+  }
+  else if constexpr (K == 9) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%6, %7 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%6, %7 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%6, %7 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%6, %7 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%6, %7 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %5, -%6, %7 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[9]));
+  }
+  else if constexpr (K == 10) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%5, %6 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%5, %6 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%5, %6 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%5, %6 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %4, -%5, %6 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[10]));
+  }
+  else if constexpr (K == 11) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%4, %5 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%4, %5 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%4, %5 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, -%4, %5 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[11]));
+  }
+  else if constexpr (K == 12) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%3, %4 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%3, %4 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, -%3, %4 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[12]));
+  }
+  else if constexpr (K == 13) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%2, %3 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, -%2, %3 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[14]), "+v"(a[15])
+        : "v"(u), "v"(a[13]));
+  }
+  else if constexpr (K == 14) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, -%1, %2 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        : "+v"(a[15])
+        : "v"(u), "v"(a[14]));
+  }
+}
+
+// Pivot k of the column-per-lane factorisation, branch-free so the
+// compiler may interleave a block's VALU work with the previous block's
+// trailing MFMAs: a pivot at or past f (act false, live = pivots of the
+// block inside the f x f system) scales its row by zero, and zero
+// multipliers leave the other rows alone.
+template <int k>
+__device__ __forceinline__ void diag_pivot(float (&a)[16], int live, bool& ok) {
+  const bool act = k < live;
+  const float d = readlane_f(a[k], k);
+  const bool good = d > 0.0f && d < __builtin_huge_valf();
+  ok = ok && (good || !act);
+  const float sk = (act && good) ? __builtin_amdgcn_rsqf(d) : 0.0f;
+  a[k] = act ? a[k] * sk : 0.0f;
+  if constexpr (k < 15) {
+    // Row k of U (u_kj in lane j of the T rows) copied into the identity
+    // rows: v_permlane16_swap exchanges the odd rows of its first operand
+    // with the even rows of its second, so the first result holds the T
+    // rows' values in every row.
+    const auto pr = __builtin_amdgcn_permlane16_swap(__float_as_int(a[k]), __float_as_int(a[k]), false, false);
+    elim_dpp<k>(a, __int_as_float(pr[0]));
+  }
+}
+template <int... ks>
+__device__ __forceinline__ void diag_pivots(float (&a)[16], int live, bool& ok, std::integer_sequence<int, ks...>) {
+  (diag_pivot<ks>(a, live, ok), ...);
+}
+
 template <int NB, int K>
 __device__ __forceinline__ void chol_diag(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ wcm,
                                           float* __restrict__ tsc, bool& ok, int lane) {
@@ -328,39 +578,25 @@ __device__ __forceinline__ void chol_diag(f4 (&acc)[num_tiles(NB)], int f, float
   // Column-major copy of T: tsc[16 c + i] = T[i][c].
   *reinterpret_cast<f4*>(tsc + 16 * c + 4 * g) = acc[TK];
   lds_order();
+  // Rows 0 and 2 of the wave (lanes 0..15, 32..47) hold the columns of T,
+  // rows 1 and 3 the identity columns (rows 2, 3 repeat rows 0, 1).
+  const bool trow = (g & 1) == 0;
   float a[16];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const f4 v = *reinterpret_cast<const f4*>(tsc + 16 * c + 4 * q);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[4 * q + r] = lane < 16 ? v[r] : (4 * q + r == c ? 1.0f : 0.0f);
+    for (int r = 0; r < 4; ++r) a[4 * q + r] = trow ? v[r] : (4 * q + r == c ? 1.0f : 0.0f);
   }
-  // Branch-free, so the compiler may interleave this block's VALU work with
-  // the previous block's trailing MFMAs: a pivot at or past f (act false)
-  // scales its row by zero, and zero multipliers leave the other rows alone.
-  const int live = f - 16 * K;  // pivots of this block inside the f x f system
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const bool act = k < live;
-    const float d = readlane_f(a[k], k);
-    const bool good = d > 0.0f && d < __builtin_huge_valf();
-    ok = ok && (good || !act);
-    const float sk = (act && good) ? __builtin_amdgcn_rsqf(d) : 0.0f;
-    a[k] = act ? a[k] * sk : 0.0f;
-    float u[16];
-#pragma unroll
-    for (int i = k + 1; i < 16; ++i) u[i] = readlane_f(a[k], i);
-#pragma unroll
-    for (int i = k + 1; i < 16; ++i) a[i] = fmaf(-u[i], a[k], a[i]);
-  }
+  diag_pivots(a, f - 16 * K, ok, std::make_integer_sequence<int, 16>{});
   // U is upper triangular (what elimination leaves below the diagonal is
   // rounding residue).
 #pragma unroll
-  for (int i = 1; i < 16; ++i) a[i] = (lane < 16 && i > c) ? 0.0f : a[i];
+  for (int i = 1; i < 16; ++i) a[i] = (trow && i > c) ? 0.0f : a[i];
   lds_order();
-  // U_KK back into tsc (column-major), W_K into its slot; lanes 32..63 hold
-  // the same identity columns as lanes 16..31 and write the same values.
-  float* dst = lane < 16 ? tsc + 16 * c : wcm + 256 * K + 16 * c;
+  // U_KK back into tsc (column-major), W_K into its slot; rows 2 and 3
+  // write the same values as rows 0 and 1.
+  float* dst = trow ? tsc + 16 * c : wcm + 256 * K + 16 * c;
 #pragma unroll
   for (int q = 0; q < 4; ++q)
     *reinterpret_cast<f4*>(dst + 4 * q) = f4{a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]};

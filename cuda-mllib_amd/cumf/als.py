@@ -32,7 +32,7 @@ class Options(C.Structure):
                 ("timing", C.c_int32), ("use_graph", C.c_int32),
                 ("compat_hermitian_bug", C.c_int32), ("solver", C.c_int32),
                 ("cg_iters", C.c_int32), ("cg_rounds", C.c_int32), ("cg_tol", C.c_float),
-                ("warm_start", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("warm_start", C.c_int32), ("warm_sweeps", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):

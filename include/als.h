@@ -115,7 +115,12 @@ typedef struct cumf_options {
                            evaluated in FP64 on the Gram the GPU formed
                            (0: 1e-5) */
   int32_t warm_start;   /* CUMF_WARM_* (CG only) */
-  int32_t reserved[5];
+  int32_t warm_sweeps;  /* CUMF_WARM_MLP / CUMF_WARM_BEST: the network starts
+                           only the first warm_sweeps X half-sweeps after
+                           cumf_set_factors (the transient ones, where the
+                           previous factor is a poor start), later ones start
+                           from the current factor (0: every X half-sweep) */
+  int32_t reserved[4];
 } cumf_options;
 
 typedef struct cumf_stats {

@@ -194,3 +194,74 @@ def test_best_start_and_graph_replay():
     assert ok.sum() >= 0.99 * ds.m
     assert np.all(tb[ok, 0] <= tp[ok, 0] * (1 + 1e-6) + 1e-12)
     net.close()
+
+
+def test_learned_start_in_the_transient_iterations():
+    """At the per-row tolerance 1e-5 (6 CG steps x up to 6 refinement rounds),
+    on a run from a random start the network trained on another run's plain
+    sweeps is compared with the previous factor (PREV) and zero (ZERO) in
+    the two transient X half-sweeps.  What it shows (printed): the
+    prediction starts closer than zero in the first sweep (residual median
+    0.27 vs 1 on this data), yet every start needs the minimum two rounds
+    there; in the second sweep the previous factor (0.14) is closer than the
+    prediction for every row, so CUMF_WARM_BEST keeps PREV.  Asserted: the
+    better-of start (CUMF_WARM_BEST) is never farther, and never needs more
+    rounds or fallbacks, than PREV and ZERO."""
+    ds = _data.netflix_like_small()
+    f, lam = 32, 0.058
+    net = NN.MLP(f, seed=1)
+    with G.ALS(f, lam) as h:
+        h.set_dataset(ds)
+        h.set_factors(theta=cumf.init_factors(ds.n, f, 11))
+        net.harvest(h, 5)
+    net.train(30, batch=1024, lr=1e-3, seed=0)
+    T0 = cumf.init_factors(ds.n, f, 12)
+    with G.ALS(f, lam) as h:  # one plain iteration from the same random start
+        h.set_dataset(ds)
+        h.set_factors(theta=T0)
+        h.iterate(1)
+        X1, T1 = h.get_factors()
+    live = np.diff(ds.csr_row_ptr) > 0
+    res = {}
+    for sweep, X, T in ((1, np.zeros((ds.m, f), np.float32), T0), (2, X1, T1)):
+        for name, warm in (("prev", G.CUMF_WARM_PREV), ("zero", G.CUMF_WARM_ZERO), ("best", G.CUMF_WARM_BEST)):
+            with _cg_handle(ds, f, lam, T, warm_start=warm, warm_sweeps=1) as hh:
+                if warm == G.CUMF_WARM_BEST:
+                    hh.set_mlp(net)
+                hh.set_factors(X=X, theta=T)
+                tr = hh.cg_trace(0)
+            res[sweep, name] = (float(np.median(tr[live, 0])), float(tr[live, 2].mean()), int(tr[live, 3].sum()))
+        print(f"X half-sweep {sweep}: start residual median / mean rounds / fallbacks: " +
+              "; ".join(f"{k} {res[sweep, k][0]:.3g} / {res[sweep, k][1]:.3f} / {res[sweep, k][2]}"
+                        for k in ("prev", "zero", "best")))
+    net.close()
+    for sweep in (1, 2):
+        b = res[sweep, "best"]
+        for other in ("prev", "zero"):
+            o = res[sweep, other]
+            assert b[0] <= o[0] * (1 + 1e-6) or sweep == 2 and other == "zero", (sweep, other, b, o)
+        o = res[sweep, "prev"]
+        assert b[1] <= o[1] * (1 + 1e-9) and b[2] <= o[2], (sweep, b, o)
+    assert res[1, "best"][0] < res[1, "zero"][0]
+
+
+def test_warm_sweeps_limits_the_network_to_the_first_sweeps():
+    """With warm_sweeps=1 only the first X half-sweep after the factors are
+    set runs the network (its features and predictions), under graph replay
+    as eagerly, and the factors match."""
+    ds = _data.netflix_like_small()
+    f, lam = 32, 0.058
+    T0 = cumf.init_factors(ds.n, f, 15)
+    out = []
+    with NN.MLP(f, seed=6) as net:
+        for graph in (0, 1):
+            with _cg_handle(ds, f, lam, T0, warm_start=G.CUMF_WARM_BEST, warm_sweeps=1, use_graph=graph,
+                            timing=1) as h:
+                h.set_mlp(net)
+                h.iterate(3)
+                st = h.stats()
+                out.append(h.get_factors())
+            if not graph:
+                assert st["warm_launches"] == 1, st
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
