@@ -87,7 +87,8 @@ __device__ __forceinline__ void run_task(const SweepArgs& a, const Task& t, floa
 }
 
 // tasks[0 .. ntasks): one work-group per task, or (PERSISTENT) a grid of
-// wavefronts that take tasks from a counter until none are left.  Separate
+// wavefronts that take tasks from a counter until none are left (with no
+// task list, PERSISTENT takes the rows 0 .. ntasks themselves).  Separate
 // instantiations: the loop changes register allocation (more spill), so the
 // one-task-per-group kernel keeps its own.
 template <int NB, bool BF, bool CG, bool PERSISTENT>
@@ -103,7 +104,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
       idx = __builtin_amdgcn_readfirstlane(idx);
       if (idx >= a.ntasks) break;
       lds_order();
-      run_task<NB, BF, CG>(a, a.tasks[idx], lds, lane);
+      // Without a task list the tasks are the rows themselves, whole.
+      const Task t = a.tasks ? a.tasks[idx] : Task{idx, a.row_ptr[idx], a.row_ptr[idx + 1] - a.row_ptr[idx], -1};
+      run_task<NB, BF, CG>(a, t, lds, lane);
     }
   }
 }

@@ -187,6 +187,22 @@ const char* cumf_last_error(void);
 int cumf_half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const int32_t* d_col,
                            const float* d_val, const float* d_theta, float* d_x, int64_t m,
                            int64_t n, int f, int64_t nnz, float lambda, const cumf_options* opt);
+/* The same half-sweep for callers that cannot allocate or synchronise
+ * (code under HIP graph capture, tight serving loops); no counterpart in
+ * the reference.  FP32 Gram + FP32 Cholesky like cumf_half_sweep_device,
+ * but: no device allocation, no host readback and no synchronisation (the
+ * work is enqueued on `stream`); whole rows taken by a persistent grid from
+ * a counter, so a very long row is solved by one wavefront (no split); the
+ * CSR must be consistent (row_ptr[0] = 0, row_ptr[m] = nnz, columns < n),
+ * which is not checked on the host.  Limits: f a multiple of 4 in [4, 140]
+ * (factor rows are read in place, row stride f), m <= INT_MAX - 2^20.  Its
+ * only scratch memory is a 256-byte device array per GPU (the row counter
+ * and the failed-row count), held until the process ends and shared by
+ * every call on that GPU: calls that overlap on one GPU race.  Rows with a
+ * non-positive pivot get x_u = 0. */
+int cumf_half_sweep_device_noalloc(hipStream_t stream, const int32_t* d_row_ptr, const int32_t* d_col,
+                                   const float* d_val, const float* d_theta, float* d_x, int64_t m, int64_t n,
+                                   int f, int64_t nnz, float lambda);
 
 /* ------------------------------------------------------------------------
  * Rating-matrix ingestion.
