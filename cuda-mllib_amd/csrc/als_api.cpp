@@ -183,7 +183,6 @@ struct cumf_handle {
   DevBuf rp, ci, rv, cp, ri, cv;  // CSR and CSC
   DevBuf X, T;                    // factors, row stride ldf
   DevBuf Xb, Tb;                  // BF16 shadow factors (precision mode), row stride ldb
-  DevBuf zero16;                  // 64 zero bytes
   int64_t ldb = 0;
   DevBuf tu, tv, tr;              // held-out COO
   DevBuf rmse_partial, rmse_out, fail;
@@ -376,7 +375,6 @@ int fill_args(cumf_handle* h, const Plan& p, const DevBuf& rowp, const DevBuf& c
   a.ldb = h->ldb;
   a.theta_b = theta_b.as<uint16_t>();
   a.x_b = x_b.as<uint16_t>();
-  a.zero16 = h->zero16.p;
   a.solver = h->opt.solver;
   a.cg_iters = h->opt.cg_iters;
   a.cg_rounds = h->opt.cg_rounds;
@@ -586,8 +584,6 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
   }
   if (rc == CUMF_OK) rc = h->rmse_partial.alloc(sizeof(double) * kRmseBlocks);
   if (rc == CUMF_OK) rc = h->rmse_out.alloc(sizeof(double));
-  if (rc == CUMF_OK) rc = h->zero16.alloc(64);
-  if (rc == CUMF_OK && hipMemset(h->zero16.p, 0, 64) != hipSuccess) rc = CUMF_ERR_HIP;
   if (rc == CUMF_OK && hipMemset(h->fail.p, 0, 2 * sizeof(int)) != hipSuccess) rc = CUMF_ERR_HIP;
   if (rc == CUMF_OK && f > kMaxFusedF && cumf::prepare_large_kernel(f) != hipSuccess) {
     (void)hipGetLastError();
@@ -993,7 +989,6 @@ int cumf_debug_row_gram(cumf_handle* h, int side, const int32_t* rows, int64_t n
   a.precision = h->opt.precision;
   a.ldb = h->ldb;
   a.theta_b = side == 0 ? h->Tb.as<uint16_t>() : h->Xb.as<uint16_t>();
-  a.zero16 = h->zero16.p;
   hipError_t e = cumf::launch_dump_kernel(a, drows.as<int32_t>(), nrows, dout.as<float>(), h->stream);
   if (e != hipSuccess) {
     cumf::set_last_error(std::string("dump kernel: ") + hipGetErrorString(e));

@@ -48,14 +48,14 @@ __host__ __device__ constexpr int tile_index(int I, int J, int NB) {
   return I * NB - I * (I - 1) / 2 + (J - I);
 }
 
-// LDS floats per wave: the staging buffers of the Gram phase and the
-// buffers of the Cholesky solve share the same space.
+// LDS floats per wave: the staging buffers of the Gram phase (FP32: two
+// chunks of NB tiles of 16 x 16 floats plus ratings; BF16: three chunks of
+// NB tiles of 32 x 16 bf16) and the buffers of the Cholesky solve (W blocks,
+// diagonal tile, identity, z, z2) share the same space.
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
-__host__ __device__ constexpr int chol_lds_floats(int NB) { return NB * 256 + 256 + 32 * NB; }
-__host__ __device__ constexpr int lds_floats_nb(int NB, int f) {
-  return cmax(cmax(2 * KB * ((f + 3) & ~3) + 2 * KB,  // FP32 staging
-                   3 * 32 * 16 * NB / 2),             // BF16 staging (3 x 32 rows x 16NB bf16)
-              chol_lds_floats(NB));                   // W blocks, diagonal tile, z, z2
+__host__ __device__ constexpr int chol_lds_floats(int NB) { return NB * 256 + 512 + 32 * NB; }
+__host__ __device__ constexpr int lds_floats_nb(int NB, bool bf) {
+  return cmax(bf ? 3 * NB * 256 : 2 * NB * 256 + 2 * KB, chol_lds_floats(NB));
 }
 
 
@@ -107,15 +107,25 @@ __device__ __forceinline__ void lds_order() {
   asm volatile("" ::: "memory");
 }
 
+// Zeros that gathers read in place of factor rows past the end of a range
+// (and of columns past the end of a factor row): 1 KB, never written.
+static __device__ const float kZeroRow[256] = {};
+
 // ---------------------------------------------------------------------------
 // Gram accumulation over a contiguous range of one row's ratings.
 //
-// Staging: two LDS buffers of KB factor rows (row-major, stride ldf) plus
-// KB ratings.  Factor rows are gathered straight into LDS with
-// global_load_lds_dwordx4 (lane l of load `it` moves 16 bytes of slot
-// it*64+l = (row, quad) to LDS offset 16*(it*64+l), i.e. row*ldf+4*quad),
-// one chunk ahead of the MFMAs.  Rows past the end of the range are
-// zeroed when the MFMA operands are read.
+// Staging: two LDS buffers per chunk of KB = 16 ratings, each NB tiles of
+// 16 rows x 16 floats (tile I holds columns 16I .. 16I+15 of the chunk's
+// factor rows, row stride 64 bytes), plus the KB ratings.  Load I of a
+// chunk fills tile I: lane l moves the 16 bytes of row l/4, columns
+// 16I + 4(l%4) .. +3, with global_load_lds_dwordx4 to byte 16l of the tile,
+// so every lane serves one fixed row of the chunk and loads that row's
+// column index itself (no cross-lane exchange).  sweep_nb(f) gives
+// 16 (NB-1) <= ldf < 16 NB: tiles 0 .. NB-2 lie inside the factor row, the
+// last tile holds the rest of it, the rating column (ldf) and padding.  In
+// the last tile, quads at or past ldf -- and every quad of a row past the
+// end of the range -- come from kZeroRow; the rating column is fed to the
+// MFMA from the rating buffer.  Gathers run one chunk ahead of the MFMAs.
 // ---------------------------------------------------------------------------
 template <int NB>
 __device__ __forceinline__ void gram_range(f4 (&acc)[num_tiles(NB)], const int32_t* __restrict__ col,
@@ -124,60 +134,47 @@ __device__ __forceinline__ void gram_range(f4 (&acc)[num_tiles(NB)], const int32
                                            int64_t start, int count, float* __restrict__ lds,
                                            int lane) {
   const int g = lane >> 4, c = lane & 15;
-  const int ld = (int)ldf;
-  const int Q = ld >> 2;       // 16-byte quads per factor row
-  const int slots = KB * Q;    // quads per chunk
-  const int cf = KB * ld;      // floats per staged chunk
-  float* rb = lds + 2 * cf;    // ratings, [2][KB]
-  // Last column block: factor columns, then the rating column (ldf), then zeros.
-  const int lastc = 16 * (NB - 1) + c;
-  const int lastkind = lastc < ld ? 0 : (lastc == ld ? 1 : 2);
-  const int lastoff = lastc < ld ? lastc : ld - 1;
+  const int sr = lane >> 2, sq = lane & 3;  // staging row and quad of this lane
+  constexpr int CF = NB * 256;              // floats per staged chunk
+  float* rb = lds + 2 * CF;                 // ratings, [2][KB]
+  const bool last_in = 16 * (NB - 1) + 4 * sq < (int)ldf;
+  const bool rating_lane = 16 * (NB - 1) + c == (int)ldf;
   const int nch = (count + KB - 1) / KB;
 
+  // Column index of this lane's staging row in chunk ch (-1 past the end).
   auto load_ci = [&](int ch) -> int {
-    const int k = ch * KB + lane;
-    return (lane < KB && ch < nch && k < count) ? col[start + k] : 0;
+    const int k = ch * KB + sr;
+    return (ch < nch && k < count) ? col[start + k] : -1;
   };
   auto issue = [&](int ch, int ci) {
-    float* yb = lds + (ch & 1) * cf;
+    float* yb = lds + (ch & 1) * CF;
+    const float* rp = ci >= 0 ? theta + (int64_t)ci * ldf + 4 * sq : kZeroRow;
 #pragma unroll
-    for (int it = 0; it < NB; ++it) {
-      if (it * 64 < slots) {
-        const int sl = it * 64 + lane;
-        const int row = sl / Q;
-        const int q = sl - row * Q;
-        const int src = __shfl(ci, row < KB ? row : 0);
-        if (sl < slots)
-          __builtin_amdgcn_global_load_lds(theta + (int64_t)src * ldf + 4 * q,
-                                           (lds_void_t*)(yb + it * 256), 16, 0, 0);
-      }
-    }
+    for (int I = 0; I < NB - 1; ++I)
+      __builtin_amdgcn_global_load_lds(rp + 16 * I, (lds_void_t*)(yb + I * 256), 16, 0, 0);
+    const float* rl = last_in ? rp + 16 * (NB - 1) : kZeroRow;
+    __builtin_amdgcn_global_load_lds(rl, (lds_void_t*)(yb + (NB - 1) * 256), 16, 0, 0);
     const int k = ch * KB + lane;
     if (lane < KB && k < count)
       __builtin_amdgcn_global_load_lds(val + start + k, (lds_void_t*)(rb + (ch & 1) * KB), 4, 0, 0);
   };
 
-  int ci_next = load_ci(0);
-  issue(0, ci_next);
-  ci_next = load_ci(1);
+  issue(0, load_ci(0));
+  int ci_next = load_ci(1);
   for (int ch = 0; ch < nch; ++ch) {
-    wait_vm_all();  // chunk ch is in LDS; column indices of ch+1 are in ci_next
+    wait_vm_all();  // chunk ch is in LDS; the column index of ch+1 is in ci_next
     if (ch + 1 < nch) issue(ch + 1, ci_next);
     ci_next = load_ci(ch + 2);
-    const float* yb = lds + (ch & 1) * cf;
+    const float* yb = lds + (ch & 1) * CF;
     const float* rr = rb + (ch & 1) * KB;
     const int valid = count - ch * KB;
 #pragma unroll
     for (int s = 0; s < KB / 4; ++s) {
       const int row = 4 * s + g;
-      const bool ok = row < valid;
       float a[NB];
 #pragma unroll
-      for (int I = 0; I < NB - 1; ++I) a[I] = ok ? yb[row * ld + 16 * I + c] : 0.0f;
-      const float yl = yb[row * ld + lastoff];
-      const float rl = rr[row];
-      a[NB - 1] = !ok ? 0.0f : (lastkind == 0 ? yl : (lastkind == 1 ? rl : 0.0f));
+      for (int I = 0; I < NB; ++I) a[I] = yb[I * 256 + row * 16 + c];
+      if (rating_lane) a[NB - 1] = row < valid ? rr[row] : 0.0f;
 #pragma unroll
       for (int I = 0; I < NB; ++I)
 #pragma unroll
@@ -195,40 +192,46 @@ __device__ __forceinline__ void gram_range(f4 (&acc)[num_tiles(NB)], const int32
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
 //
 // Factor rows come from a BF16 shadow copy (row stride ldb = round_up(f,8),
-// zero padded).  A chunk of 32 ratings is staged in LDS as 32 rows of
-// W = 16*NB bf16: 2*NB 16-byte slots per row, moved with
-// global_load_lds_dwordx4 (slots past the factor row, and every slot of a
-// row past the end of the range, come from a zero block).  The rating r is
-// written into columns f and f+1 as bf16(r) and bf16(r - bf16(r)), so
-// b = Y^T r keeps ~16 significant bits (exact for integer ratings).
+// zero padded).  A chunk of 32 ratings is staged in LDS as NB tiles of
+// 32 rows x 16 bf16 (row stride 32 bytes).  Load I of a chunk fills tile I:
+// lane l moves the 16 bytes of row l/2, columns 16I + 8(l%2) .. +7, with
+// global_load_lds_dwordx4 to byte 16l of the tile; each lane loads its own
+// row's column index.  sweep_nb(f) gives 16 (NB-1) <= ldb, so tiles
+// 0 .. NB-2 lie inside the factor row; in the last tile, slots at or past
+// ldb -- and every slot of a row past the end of the range -- come from
+// kZeroRow.  The rating r is then written into columns f and f+1 as bf16(r)
+// and bf16(r - bf16(r)), so b = Y^T r keeps ~16 significant bits (exact for
+// integer ratings).
 //
-// MFMA operands come straight from the row-major tile with
-// ds_read_b64_tr_b16: within a 16-lane group, lane j addresses 4 bf16 of
-// row (8*kg + j/4), columns 16I + 4*(j%4); the transpose read returns to
-// lane i the element (i%4) of the segment of lane 4s + i/4, i.e. column
-// 16I + i of rows 8kg + s, s = 0..3.  A second read 4 rows further gives
-// the 8 consecutive k values the 16x16x32 operand needs.
+// MFMA operands come straight from the tiles with ds_read_b64_tr_b16:
+// within a 16-lane group, lane j addresses 4 bf16 of row (8*kg + j/4),
+// columns 4*(j%4) of tile I; the transpose read returns to lane i the
+// element (i%4) of the segment of lane 4s + i/4, i.e. column 16I + i of
+// rows 8kg + s, s = 0..3.  A second read 4 rows further gives the 8
+// consecutive k values the 16x16x32 operand needs.
 // ---------------------------------------------------------------------------
 constexpr int KBB = 32;  // ratings per BF16 chunk (one MFMA k-step)
 
 template <int NB>
 __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const int32_t* __restrict__ col,
                                                 const float* __restrict__ val,
-                                                const __bf16* __restrict__ theta_b, int64_t ldb,
-                                                const void* __restrict__ zero16, int f, int64_t start,
-                                                int count, __bf16* __restrict__ lds, int lane) {
-  constexpr int W = 16 * NB;        // bf16 per staged row
-  constexpr int SPR = 2 * NB;       // 16-byte slots per staged row
-  constexpr int CB = KBB * W;       // bf16 per staged chunk
-  const int real = (int)(ldb >> 3);  // slots carrying factor data
+                                                const __bf16* __restrict__ theta_b, int64_t ldb, int f,
+                                                int64_t start, int count, __bf16* __restrict__ lds, int lane) {
+  constexpr int TB = KBB * 16;   // bf16 per staged tile
+  constexpr int CB = NB * TB;    // bf16 per staged chunk
+  const int sr = lane >> 1, sh = lane & 1;  // staging row and half of this lane
+  const bool last_in = 16 * (NB - 1) + 8 * sh < (int)ldb;
+  const __bf16* zb = reinterpret_cast<const __bf16*>(kZeroRow);
   const int kg = lane >> 4, j = lane & 15;
   const int nch = (count + KBB - 1) / KBB;
-  // this lane's transpose-read offsets inside a chunk (bf16 units)
-  const int troff = (8 * kg + (j >> 2)) * W + 4 * (j & 3);
+  // this lane's transpose-read offset inside a tile (bf16 units)
+  const int troff = (8 * kg + (j >> 2)) * 16 + 4 * (j & 3);
+  // where the rating halves go: tile and column of f and f+1
+  const int rt0 = (f >> 4) * TB + (f & 15), rt1 = ((f + 1) >> 4) * TB + ((f + 1) & 15);
 
   auto load_ci = [&](int ch) -> int {
-    const int k = ch * KBB + lane;
-    return (lane < KBB && ch < nch && k < count) ? col[start + k] : -1;
+    const int k = ch * KBB + sr;
+    return (ch < nch && k < count) ? col[start + k] : -1;
   };
   auto load_r = [&](int ch) -> float {
     const int k = ch * KBB + lane;
@@ -236,15 +239,12 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
   };
   auto issue = [&](int ch, int ci) {
     __bf16* yb = lds + (ch % 3) * CB;
+    const __bf16* rp = ci >= 0 ? theta_b + (int64_t)ci * ldb + 8 * sh : zb;
 #pragma unroll
-    for (int it = 0; it < NB; ++it) {
-      const int sl = it * 64 + lane;  // NB*64 = KBB*SPR slots exactly
-      const int row = sl / SPR;
-      const int q = sl - row * SPR;
-      const int src = __shfl(ci, row);
-      const void* g = (src >= 0 && q < real) ? (const void*)(theta_b + (int64_t)src * ldb + 8 * q) : zero16;
-      __builtin_amdgcn_global_load_lds(g, (lds_void_t*)(yb + it * 512), 16, 0, 0);
-    }
+    for (int I = 0; I < NB - 1; ++I)
+      __builtin_amdgcn_global_load_lds(rp + 16 * I, (lds_void_t*)(yb + I * TB), 16, 0, 0);
+    const __bf16* rl = last_in ? rp + 16 * (NB - 1) : zb;
+    __builtin_amdgcn_global_load_lds(rl, (lds_void_t*)(yb + (NB - 1) * TB), 16, 0, 0);
   };
 
   // Gathers run two chunks ahead (three LDS buffers); column indices and
@@ -266,8 +266,8 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
     if (lane < KBB && ch * KBB + lane < count) {
       const __bf16 hi = (__bf16)rv;
       const __bf16 lo = (__bf16)(rv - (float)hi);
-      yb[lane * W + f] = hi;
-      yb[lane * W + f + 1] = lo;
+      yb[rt0 + 16 * lane] = hi;
+      yb[rt1 + 16 * lane] = lo;
     }
     if (ch + 2 < nch) issue(ch + 2, ci2);
     asm volatile("" ::: "memory");  // keep the next loads after the gathers
@@ -281,8 +281,8 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
     bf8 op[NB];
 #pragma unroll
     for (int I = 0; I < NB; ++I) {
-      const bf4 lo4 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf4_t*)(yb + troff + 16 * I));
-      const bf4 hi4 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf4_t*)(yb + troff + 4 * W + 16 * I));
+      const bf4 lo4 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf4_t*)(yb + I * TB + troff));
+      const bf4 hi4 = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf4_t*)(yb + I * TB + troff + 64));
       op[I] = __builtin_shufflevector(lo4, hi4, 0, 1, 2, 3, 4, 5, 6, 7);
     }
 #pragma unroll
@@ -322,7 +322,8 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
 // by block from the last: x_K = W_K^T (z_K - sum_{J > K} U_KJ x_J).
 //
 // LDS (floats): W_K for every block (NB x 256), the diagonal tile being
-// factored (256), z and the second right-hand side's z (16 NB each).
+// factored (256), a 16 x 16 identity (256), z and the second right-hand
+// side's z (16 NB each).
 // ---------------------------------------------------------------------------
 // Elimination step of pivot K in the column-per-lane factorisation:
 // a[i] -= u_i * a[K] for i = K+1..15, where the multiplier u_i is the value
@@ -543,52 +544,56 @@ __device__ __forceinline__ void elim_dpp(float (&a)[16], float u) {
   }
 }
 
-// Pivot k of the column-per-lane factorisation, branch-free so the
-// compiler may interleave a block's VALU work with the previous block's
-// trailing MFMAs: a pivot at or past f (act false, live = pivots of the
-// block inside the f x f system) scales its row by zero, and zero
-// multipliers leave the other rows alone.
-template <int k>
-__device__ __forceinline__ void diag_pivot(float (&a)[16], int live, bool& ok) {
+// Pivot k of the column-per-lane factorisation (live = pivots of the block
+// inside the f x f system).  A pivot at or past f scales its row by zero,
+// and zero multipliers leave the other rows alone, so the step is
+// branch-free and the compiler may interleave a block's VALU work with the
+// previous block's trailing MFMAs; only the last block (GUARD: nothing
+// follows it) skips the elimination of such pivots.  A pivot that is not
+// positive and finite turns its row into NaN or infinities, which reach
+// the solution; chol_solve tests the solution instead of every pivot.
+template <int k, bool GUARD>
+__device__ __forceinline__ void diag_pivot(float (&a)[16], int live) {
   const bool act = k < live;
   const float d = readlane_f(a[k], k);
-  const bool good = d > 0.0f && d < __builtin_huge_valf();
-  ok = ok && (good || !act);
-  const float sk = (act && good) ? __builtin_amdgcn_rsqf(d) : 0.0f;
-  a[k] = act ? a[k] * sk : 0.0f;
+  a[k] = act ? a[k] * __builtin_amdgcn_rsqf(d) : 0.0f;
   if constexpr (k < 15) {
-    // Row k of U (u_kj in lane j of the T rows) copied into the identity
-    // rows: v_permlane16_swap exchanges the odd rows of its first operand
-    // with the even rows of its second, so the first result holds the T
-    // rows' values in every row.
-    const auto pr = __builtin_amdgcn_permlane16_swap(__float_as_int(a[k]), __float_as_int(a[k]), false, false);
-    elim_dpp<k>(a, __int_as_float(pr[0]));
+    if (!GUARD || act) {
+      // Row k of U (u_kj in lane j of the T rows) copied into the identity
+      // rows: v_permlane16_swap exchanges the odd rows of its first operand
+      // with the even rows of its second, so the first result holds the T
+      // rows' values in every row.
+      const auto pr = __builtin_amdgcn_permlane16_swap(__float_as_int(a[k]), __float_as_int(a[k]), false, false);
+      elim_dpp<k>(a, __int_as_float(pr[0]));
+    }
   }
 }
-template <int... ks>
-__device__ __forceinline__ void diag_pivots(float (&a)[16], int live, bool& ok, std::integer_sequence<int, ks...>) {
-  (diag_pivot<ks>(a, live, ok), ...);
+template <bool GUARD, int... ks>
+__device__ __forceinline__ void diag_pivots(float (&a)[16], int live, std::integer_sequence<int, ks...>) {
+  (diag_pivot<ks, GUARD>(a, live), ...);
 }
 
 template <int NB, int K>
 __device__ __forceinline__ void chol_diag(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ wcm,
-                                          float* __restrict__ tsc, bool& ok, int lane) {
+                                          float* __restrict__ tsc, const float* __restrict__ ident, int lane) {
   const int g = lane >> 4, c = lane & 15;
   constexpr int TK = tile_index(K, K, NB);
   // Column-major copy of T: tsc[16 c + i] = T[i][c].
   *reinterpret_cast<f4*>(tsc + 16 * c + 4 * g) = acc[TK];
   lds_order();
   // Rows 0 and 2 of the wave (lanes 0..15, 32..47) hold the columns of T,
-  // rows 1 and 3 the identity columns (rows 2, 3 repeat rows 0, 1).
+  // rows 1 and 3 the identity columns (rows 2, 3 repeat rows 0, 1), both
+  // read from LDS.
   const bool trow = (g & 1) == 0;
+  const float* src = (trow ? tsc : ident) + 16 * c;
   float a[16];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const f4 v = *reinterpret_cast<const f4*>(tsc + 16 * c + 4 * q);
+    const f4 v = *reinterpret_cast<const f4*>(src + 4 * q);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[4 * q + r] = trow ? v[r] : (4 * q + r == c ? 1.0f : 0.0f);
+    for (int r = 0; r < 4; ++r) a[4 * q + r] = v[r];
   }
-  diag_pivots(a, f - 16 * K, ok, std::make_integer_sequence<int, 16>{});
+  diag_pivots<K + 1 == NB>(a, f - 16 * K, std::make_integer_sequence<int, 16>{});
   // U is upper triangular (what elimination leaves below the diagonal is
   // rounding residue).
 #pragma unroll
@@ -638,15 +643,15 @@ __device__ __forceinline__ void chol_trailing(f4 (&acc)[num_tiles(NB)]) {
 
 template <int NB, int K>
 __device__ __forceinline__ void chol_blocks(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ wcm,
-                                            float* __restrict__ tsc, bool& ok, int lane) {
+                                            float* __restrict__ tsc, const float* __restrict__ ident, int lane) {
   // Every block but the last runs unconditionally (one basic block for the
   // scheduler); the last may hold only the right-hand side column(s).
   if (K + 1 < NB || 16 * K < f) {
-    chol_diag<NB, K>(acc, f, wcm, tsc, ok, lane);
+    chol_diag<NB, K>(acc, f, wcm, tsc, ident, lane);
     chol_panel<NB, K>(acc, wcm, lane);
     chol_trailing<NB, K>(acc);
   }
-  if constexpr (K + 1 < NB) chol_blocks<NB, K + 1>(acc, f, wcm, tsc, ok, lane);
+  if constexpr (K + 1 < NB) chol_blocks<NB, K + 1>(acc, f, wcm, tsc, ident, lane);
 }
 
 
@@ -689,10 +694,16 @@ __device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int 
   const int g = lane >> 4, c = lane & 15;
   float* wcm = lds;
   float* tsc = lds + NB * 256;
-  float* zb = tsc + 256;
+  float* ident = tsc + 256;
+  float* zb = ident + 256;
   float* zb2 = zb + 16 * NB;
-  bool ok = true;
-  chol_blocks<NB, 0>(acc, f, wcm, tsc, ok, lane);
+  {
+    // 16 x 16 identity: lane (g, c) writes rows 4g..4g+3 of column c.
+    const f4 e = {4 * g == c ? 1.0f : 0.0f, 4 * g + 1 == c ? 1.0f : 0.0f, 4 * g + 2 == c ? 1.0f : 0.0f,
+                  4 * g + 3 == c ? 1.0f : 0.0f};
+    *reinterpret_cast<f4*>(ident + 16 * c + 4 * g) = e;
+  }
+  chol_blocks<NB, 0>(acc, f, wcm, tsc, ident, lane);
 
   // z (and the second column's z) out of the augmented column(s), which
   // sweep_nb places in the last column block (bcol, bcol2 >= 16 (NB - 1)).
@@ -718,6 +729,11 @@ __device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int 
 #pragma unroll
   for (int J = 0; J < NB; ++J) xc[J] = 0.0f;
   chol_backsub<NB, NB - 1>(acc, f, wcm, zb, lane, xc);
+  // A pivot that was not positive and finite leaves NaN or infinities in x.
+  bool fin = true;
+#pragma unroll
+  for (int K = 0; K < NB; ++K) fin = fin && fabsf(xc[K]) < __builtin_huge_valf();
+  const bool ok = !__any(!fin);
   if (g == 0) {
 #pragma unroll
     for (int K = 0; K < NB; ++K) {
@@ -1292,7 +1308,7 @@ template <int NB, bool BF>
 __device__ __forceinline__ void gram_any(f4 (&acc)[num_tiles(NB)], const SweepArgs& a, int64_t start, int count,
                                          float* lds, int lane) {
   if constexpr (BF)
-    gram_range_bf16<NB>(acc, a.col, a.val, reinterpret_cast<const __bf16*>(a.theta_b), a.ldb, a.zero16, a.f,
+    gram_range_bf16<NB>(acc, a.col, a.val, reinterpret_cast<const __bf16*>(a.theta_b), a.ldb, a.f,
                         start, count, reinterpret_cast<__bf16*>(lds), lane);
   else
     gram_range<NB>(acc, a.col, a.val, a.theta, a.ldf, start, count, lds, lane);

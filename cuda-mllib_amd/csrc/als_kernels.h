@@ -47,7 +47,6 @@ struct SweepArgs {
   int64_t ldb;
   const uint16_t* theta_b;
   uint16_t* x_b;
-  const void* zero16;      // 16 zero bytes (gather source for padding)
   // solver != 0: FP16 conjugate gradients inside FP32 refinement with an
   // FP64 residual, warm-started from x; Cholesky for rows that do not reach
   // ||b - A x|| <= cg_tol ||b|| within cg_rounds rounds of cg_iters steps.

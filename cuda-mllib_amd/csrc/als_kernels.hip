@@ -159,7 +159,7 @@ hipError_t launch_fallback_reset(int* count, int n, hipStream_t s) {
 template <int NB>
 static hipError_t launch_dump_nb(const SweepArgs& a, const int32_t* rows, int64_t nrows, float* out, hipStream_t s) {
   if (nrows <= 0) return hipSuccess;
-  const size_t shmem = sizeof(float) * (size_t)lds_floats_nb(NB, a.f);
+  const size_t shmem = sizeof(float) * (size_t)lds_floats_nb(NB, a.precision != 0);
   if (a.precision != 0)
     hipLaunchKernelGGL((als_dump_kernel<NB, true>), dim3((unsigned)nrows), dim3(64), shmem, s, a, rows, out);
   else

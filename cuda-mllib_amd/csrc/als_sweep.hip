@@ -181,7 +181,7 @@ template <int NB>
 static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which) {
   constexpr bool CG = CUMF_SWEEP_SOLVER == 1;
   const bool bf = a.precision != 0;
-  const size_t shmem = sizeof(float) * (size_t)cmax(lds_floats_nb(NB, a.f), CG ? cg_lds_floats(NB, a.f) : 0);
+  const size_t shmem = sizeof(float) * (size_t)cmax(lds_floats_nb(NB, bf), CG ? cg_lds_floats(NB, a.f) : 0);
   if (which == 0) {
     if (a.ntasks > 0) {
       if (a.persistent > 0) {
@@ -199,7 +199,7 @@ static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which) {
       hipLaunchKernelGGL((als_finish_kernel<NB, CG>), dim3((unsigned)a.nheavy), dim3(64), shmem, s, a);
   } else if constexpr (!CG) {
     // which == 3: the CG path's Cholesky fallback (lives with the Cholesky kernels)
-    const size_t shc = sizeof(float) * (size_t)lds_floats_nb(NB, a.f);
+    const size_t shc = sizeof(float) * (size_t)lds_floats_nb(NB, bf);
     if (bf) hipLaunchKernelGGL((als_fallback_kernel<NB, true>), dim3(kFallbackGrid), dim3(64), shc, s, a);
     else hipLaunchKernelGGL((als_fallback_kernel<NB, false>), dim3(kFallbackGrid), dim3(64), shc, s, a);
   } else {
