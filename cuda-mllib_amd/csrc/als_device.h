@@ -40,7 +40,13 @@ typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __attribute__((address_space(3))) bf4 lds_bf4_t;
 
-constexpr int KB = 16;  // ratings per staged chunk
+constexpr int KB = 16;  // ratings per staged FP32 chunk
+constexpr int KBB = 32;  // ratings per BF16 chunk (one MFMA k-step)
+// Staging buffers of the BF16 Gram (NB KB each): three (gathers two chunks
+// ahead) while eight wavefronts of them fit the CU's 160 KB of LDS, else two
+// (one chunk ahead) -- at NB = 7 three would leave room for seven
+// wavefronts, and the eighth is worth more than the deeper prefetch.
+__host__ __device__ constexpr int bf16_stage_bufs(int NB) { return 8 * 3 * NB <= 160 ? 3 : 2; }
 
 
 __host__ __device__ constexpr int num_tiles(int NB) { return NB * (NB + 1) / 2; }
@@ -49,13 +55,13 @@ __host__ __device__ constexpr int tile_index(int I, int J, int NB) {
 }
 
 // LDS floats per wave: the staging buffers of the Gram phase (FP32: two
-// chunks of NB tiles of 16 x 16 floats plus ratings; BF16: three chunks of
-// NB tiles of 32 x 16 bf16) and the buffers of the Cholesky solve (W blocks,
+// chunks of NB tiles of 16 x 16 floats plus ratings; BF16: bf16_stage_bufs
+// chunks of NB tiles of 32 x 16 bf16) and the buffers of the Cholesky solve (W blocks,
 // diagonal tile, identity, z, z2) share the same space.
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ constexpr int chol_lds_floats(int NB) { return NB * 256 + 512 + 32 * NB; }
 __host__ __device__ constexpr int lds_floats_nb(int NB, bool bf) {
-  return cmax(bf ? 3 * NB * 256 : 2 * NB * 256 + 2 * KB, chol_lds_floats(NB));
+  return cmax(bf ? bf16_stage_bufs(NB) * NB * 256 : 2 * NB * 256 + 2 * KB, chol_lds_floats(NB));
 }
 
 
@@ -210,7 +216,6 @@ __device__ __forceinline__ void gram_range(f4 (&acc)[num_tiles(NB)], const int32
 // rows 8kg + s, s = 0..3.  A second read 4 rows further gives the 8
 // consecutive k values the 16x16x32 operand needs.
 // ---------------------------------------------------------------------------
-constexpr int KBB = 32;  // ratings per BF16 chunk (one MFMA k-step)
 
 template <int NB>
 __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const int32_t* __restrict__ col,
@@ -219,6 +224,7 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
                                                 int64_t start, int count, __bf16* __restrict__ lds, int lane) {
   constexpr int TB = KBB * 16;   // bf16 per staged tile
   constexpr int CB = NB * TB;    // bf16 per staged chunk
+  constexpr int NBUF = bf16_stage_bufs(NB);
   const int sr = lane >> 1, sh = lane & 1;  // staging row and half of this lane
   const bool last_in = 16 * (NB - 1) + 8 * sh < (int)ldb;
   const __bf16* zb = reinterpret_cast<const __bf16*>(kZeroRow);
@@ -238,7 +244,7 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
     return (lane < KBB && ch < nch && k < count) ? val[start + k] : 0.0f;
   };
   auto issue = [&](int ch, int ci) {
-    __bf16* yb = lds + (ch % 3) * CB;
+    __bf16* yb = lds + (ch % NBUF) * CB;
     const __bf16* rp = ci >= 0 ? theta_b + (int64_t)ci * ldb + 8 * sh : zb;
 #pragma unroll
     for (int I = 0; I < NB - 1; ++I)
@@ -247,36 +253,17 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
     __builtin_amdgcn_global_load_lds(rl, (lds_void_t*)(yb + (NB - 1) * TB), 16, 0, 0);
   };
 
-  // Gathers run two chunks ahead (three LDS buffers); column indices and
-  // ratings three chunks ahead, in registers.
-  int ci1 = load_ci(1), ci2 = load_ci(2);
-  float rv = load_r(0), r1 = load_r(1), r2 = load_r(2);
-  issue(0, load_ci(0));
-  if (1 < nch) issue(1, ci1);
-  int cn = load_ci(3);
-  float rn = load_r(3);
-  for (int ch = 0; ch < nch; ++ch) {
-    // Chunk ch's gathers are older than the last NB vector-memory
-    // operations whenever the two following chunks were issued (the next
-    // chunk's NB gathers plus the index/rating loads came after them);
-    // near the end of the range wait for everything.
-    if (ch + 2 < nch) wait_vm_recent<NB>();
-    else wait_vm_all();
-    __bf16* yb = lds + (ch % 3) * CB;
+  auto chunk = [&](int ch, float r) {
+    __bf16* yb = lds + (ch % NBUF) * CB;
     if (lane < KBB && ch * KBB + lane < count) {
-      const __bf16 hi = (__bf16)rv;
-      const __bf16 lo = (__bf16)(rv - (float)hi);
+      const __bf16 hi = (__bf16)r;
+      const __bf16 lo = (__bf16)(r - (float)hi);
       yb[rt0 + 16 * lane] = hi;
       yb[rt1 + 16 * lane] = lo;
     }
-    if (ch + 2 < nch) issue(ch + 2, ci2);
-    asm volatile("" ::: "memory");  // keep the next loads after the gathers
-    rv = r1;
-    r1 = r2;
-    r2 = rn;
-    ci2 = cn;
-    cn = load_ci(ch + 4);
-    rn = load_r(ch + 4);
+    return yb;
+  };
+  auto mfmas = [&](const __bf16* yb) {
     lds_order();
     bf8 op[NB];
 #pragma unroll
@@ -292,6 +279,51 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
         acc[tile_index(I, J, NB)] =
             __builtin_amdgcn_mfma_f32_16x16x32_bf16(op[I], op[J], acc[tile_index(I, J, NB)], 0, 0, 0);
     lds_order();
+  };
+
+  if constexpr (NBUF == 3) {
+    // Gathers run two chunks ahead; column indices and ratings three
+    // chunks ahead, in registers.
+    int ci1 = load_ci(1), ci2 = load_ci(2);
+    float rv = load_r(0), r1 = load_r(1), r2 = load_r(2);
+    issue(0, load_ci(0));
+    if (1 < nch) issue(1, ci1);
+    int cn = load_ci(3);
+    float rn = load_r(3);
+    for (int ch = 0; ch < nch; ++ch) {
+      // Chunk ch's gathers are older than the last NB vector-memory
+      // operations whenever the two following chunks were issued (the next
+      // chunk's NB gathers plus the index/rating loads came after them);
+      // near the end of the range wait for everything.
+      if (ch + 2 < nch) wait_vm_recent<NB>();
+      else wait_vm_all();
+      const __bf16* yb = chunk(ch, rv);
+      if (ch + 2 < nch) issue(ch + 2, ci2);
+      asm volatile("" ::: "memory");  // keep the next loads after the gathers
+      rv = r1;
+      r1 = r2;
+      r2 = rn;
+      ci2 = cn;
+      cn = load_ci(ch + 4);
+      rn = load_r(ch + 4);
+      mfmas(yb);
+    }
+  } else {
+    // Gathers run one chunk ahead; column indices and ratings two chunks
+    // ahead, in registers.
+    int ci1 = load_ci(1);
+    float rv = load_r(0), r1 = load_r(1);
+    issue(0, load_ci(0));
+    for (int ch = 0; ch < nch; ++ch) {
+      wait_vm_all();  // chunk ch is in LDS; indices and ratings of ch+1 are in registers
+      const __bf16* yb = chunk(ch, rv);
+      if (ch + 1 < nch) issue(ch + 1, ci1);
+      asm volatile("" ::: "memory");
+      rv = r1;
+      ci1 = load_ci(ch + 2);
+      r1 = load_r(ch + 2);
+      mfmas(yb);
+    }
   }
   wait_vm_all();
 }
