@@ -118,7 +118,8 @@ __device__ __forceinline__ void lds_order() {
 static __device__ const float kZeroRow[256] = {};
 
 // ---------------------------------------------------------------------------
-// Gram accumulation over a contiguous range of one row's ratings.
+// Gram of a contiguous range of one row's ratings (count >= 1): acc is set
+// to the range's augmented Gram, not added to.
 //
 // Staging: two LDS buffers per chunk of KB = 16 ratings, each NB tiles of
 // 16 rows x 16 floats (tile I holds columns 16I .. 16I+15 of the chunk's
@@ -165,15 +166,9 @@ __device__ __forceinline__ void gram_range(f4 (&acc)[num_tiles(NB)], const int32
       __builtin_amdgcn_global_load_lds(val + start + k, (lds_void_t*)(rb + (ch & 1) * KB), 4, 0, 0);
   };
 
-  issue(0, load_ci(0));
-  int ci_next = load_ci(1);
-  for (int ch = 0; ch < nch; ++ch) {
-    wait_vm_all();  // chunk ch is in LDS; the column index of ch+1 is in ci_next
-    if (ch + 1 < nch) issue(ch + 1, ci_next);
-    ci_next = load_ci(ch + 2);
-    const float* yb = lds + (ch & 1) * CF;
-    const float* rr = rb + (ch & 1) * KB;
-    const int valid = count - ch * KB;
+  // The MFMAs of one staged chunk; the first chunk of the range starts the
+  // accumulators from zero (an inline-constant accumulator operand).
+  auto mfma_chunk = [&](const float* yb, const float* rr, int valid, auto first) {
 #pragma unroll
     for (int s = 0; s < KB / 4; ++s) {
       const int row = 4 * s + g;
@@ -186,8 +181,22 @@ __device__ __forceinline__ void gram_range(f4 (&acc)[num_tiles(NB)], const int32
 #pragma unroll
         for (int J = I; J < NB; ++J)
           acc[tile_index(I, J, NB)] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-              a[I], a[J], acc[tile_index(I, J, NB)], 0, 0, 0);
+              a[I], a[J], (decltype(first)::value && s == 0) ? f4{0.f, 0.f, 0.f, 0.f} : acc[tile_index(I, J, NB)],
+              0, 0, 0);
     }
+  };
+
+  issue(0, load_ci(0));
+  int ci_next = load_ci(1);
+  for (int ch = 0; ch < nch; ++ch) {
+    wait_vm_all();  // chunk ch is in LDS; the column index of ch+1 is in ci_next
+    if (ch + 1 < nch) issue(ch + 1, ci_next);
+    ci_next = load_ci(ch + 2);
+    const float* yb = lds + (ch & 1) * CF;
+    const float* rr = rb + (ch & 1) * KB;
+    const int valid = count - ch * KB;
+    if (ch == 0) mfma_chunk(yb, rr, valid, std::true_type{});
+    else mfma_chunk(yb, rr, valid, std::false_type{});
     lds_order();
   }
   wait_vm_all();
@@ -263,7 +272,9 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
     }
     return yb;
   };
-  auto mfmas = [&](const __bf16* yb) {
+  // The MFMAs of one staged chunk; the first chunk of the range starts the
+  // accumulators from zero (an inline-constant accumulator operand).
+  auto mfmas = [&](const __bf16* yb, auto first) {
     lds_order();
     bf8 op[NB];
 #pragma unroll
@@ -276,9 +287,13 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
     for (int I = 0; I < NB; ++I)
 #pragma unroll
       for (int J = I; J < NB; ++J)
-        acc[tile_index(I, J, NB)] =
-            __builtin_amdgcn_mfma_f32_16x16x32_bf16(op[I], op[J], acc[tile_index(I, J, NB)], 0, 0, 0);
+        acc[tile_index(I, J, NB)] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            op[I], op[J], decltype(first)::value ? f4{0.f, 0.f, 0.f, 0.f} : acc[tile_index(I, J, NB)], 0, 0, 0);
     lds_order();
+  };
+  auto mfmas_any = [&](int ch, const __bf16* yb) {
+    if (ch == 0) mfmas(yb, std::true_type{});
+    else mfmas(yb, std::false_type{});
   };
 
   if constexpr (NBUF == 3) {
@@ -306,7 +321,7 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
       ci2 = cn;
       cn = load_ci(ch + 4);
       rn = load_r(ch + 4);
-      mfmas(yb);
+      mfmas_any(ch, yb);
     }
   } else {
     // Gathers run one chunk ahead; column indices and ratings two chunks
@@ -322,7 +337,7 @@ __device__ __forceinline__ void gram_range_bf16(f4 (&acc)[num_tiles(NB)], const 
       rv = r1;
       ci1 = load_ci(ch + 2);
       r1 = load_r(ch + 2);
-      mfmas(yb);
+      mfmas_any(ch, yb);
     }
   }
   wait_vm_all();
@@ -581,14 +596,16 @@ __device__ __forceinline__ void elim_dpp(float (&a)[16], float u) {
 // and zero multipliers leave the other rows alone, so the step is
 // branch-free and the compiler may interleave a block's VALU work with the
 // previous block's trailing MFMAs; only the last block (GUARD: nothing
-// follows it) skips the elimination of such pivots.  A pivot that is not
+// follows it) skips the elimination of such pivots.  PARTIAL: the block may
+// hold such pivots (without it every pivot is live and the select goes
+// away).  A pivot that is not
 // positive and finite turns its row into NaN or infinities, which reach
 // the solution; chol_solve tests the solution instead of every pivot.
-template <int k, bool GUARD>
+template <int k, bool PARTIAL, bool GUARD>
 __device__ __forceinline__ void diag_pivot(float (&a)[16], int live) {
-  const bool act = k < live;
+  const bool act = !PARTIAL || k < live;
   const float d = readlane_f(a[k], k);
-  a[k] = act ? a[k] * __builtin_amdgcn_rsqf(d) : 0.0f;
+  a[k] = PARTIAL ? (act ? a[k] * __builtin_amdgcn_rsqf(d) : 0.0f) : a[k] * __builtin_amdgcn_rsqf(d);
   if constexpr (k < 15) {
     if (!GUARD || act) {
       // Row k of U (u_kj in lane j of the T rows) copied into the identity
@@ -600,9 +617,9 @@ __device__ __forceinline__ void diag_pivot(float (&a)[16], int live) {
     }
   }
 }
-template <bool GUARD, int... ks>
+template <bool PARTIAL, bool GUARD, int... ks>
 __device__ __forceinline__ void diag_pivots(float (&a)[16], int live, std::integer_sequence<int, ks...>) {
-  (diag_pivot<ks, GUARD>(a, live), ...);
+  (diag_pivot<ks, PARTIAL, GUARD>(a, live), ...);
 }
 
 template <int NB, int K>
@@ -625,11 +642,16 @@ __device__ __forceinline__ void chol_diag(f4 (&acc)[num_tiles(NB)], int f, float
 #pragma unroll
     for (int r = 0; r < 4; ++r) a[4 * q + r] = v[r];
   }
-  diag_pivots<K + 1 == NB>(a, f - 16 * K, std::make_integer_sequence<int, 16>{});
-  // U is upper triangular (what elimination leaves below the diagonal is
-  // rounding residue).
+  // Only the last two blocks can hold pivots at or past f (sweep_nb gives
+  // 16 (NB - 1) <= f + 6, so f - 16 K >= 26 for K <= NB - 3).
+  diag_pivots<K + 2 >= NB, K + 1 == NB>(a, f - 16 * K, std::make_integer_sequence<int, 16>{});
+  if constexpr (K + 1 == NB) {
+    // The last block's U_KK goes back to the accumulators (its augmented
+    // columns hold z), upper triangular: what elimination leaves below the
+    // diagonal is rounding residue.  Earlier blocks' U_KK is not read again.
 #pragma unroll
-  for (int i = 1; i < 16; ++i) a[i] = (trow && i > c) ? 0.0f : a[i];
+    for (int i = 1; i < 16; ++i) a[i] = (trow && i > c) ? 0.0f : a[i];
+  }
   lds_order();
   // U_KK back into tsc (column-major), W_K into its slot; rows 2 and 3
   // write the same values as rows 0 and 1.
@@ -638,7 +660,7 @@ __device__ __forceinline__ void chol_diag(f4 (&acc)[num_tiles(NB)], int f, float
   for (int q = 0; q < 4; ++q)
     *reinterpret_cast<f4*>(dst + 4 * q) = f4{a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]};
   lds_order();
-  acc[TK] = *reinterpret_cast<const f4*>(tsc + 16 * c + 4 * g);
+  if constexpr (K + 1 == NB) acc[TK] = *reinterpret_cast<const f4*>(tsc + 16 * c + 4 * g);
 }
 
 template <int NB, int K>

@@ -48,22 +48,33 @@ namespace CUMF_SNS {
 // row goes to scratch, or the row is solved.
 template <int NB, bool BF, bool CG>
 __device__ __forceinline__ void run_task(const SweepArgs& a, const Task& t, float* lds, int lane) {
-  f4 acc[num_tiles(NB)];
-#pragma unroll
-  for (int i = 0; i < num_tiles(NB); ++i) acc[i] = f4{0.f, 0.f, 0.f, 0.f};
+  float* xrow = a.x + (int64_t)t.row * a.ldf;
+  __bf16* xb = BF ? reinterpret_cast<__bf16*>(a.x_b) + (int64_t)t.row * a.ldb : nullptr;
+  if (t.count <= 0) {
+    if (t.slot >= 0) {  // (the planner never makes an empty chunk of a heavy row)
+      float* dst = a.partial + (int64_t)t.slot * num_tiles(NB) * 256;
+      for (int i = lane; i < num_tiles(NB) * 256; i += 64) dst[i] = 0.0f;
+      return;
+    }
+    // no ratings: x_u = 0
+    for (int i = lane; i < a.ldf; i += 64) xrow[i] = 0.0f;
+    if (xb)
+      for (int i = lane; i < a.ldb; i += 64) xb[i] = (__bf16)0.0f;
+    return;
+  }
 
   // The row's current factor (the CG start) is fetched before the Gram so
   // that its latency hides behind the gathers.
   float xpre[rows_per_lane(NB)];
   if constexpr (CG) {
-    const float* xr = a.x + (int64_t)t.row * a.ldf;
 #pragma unroll
     for (int q = 0; q < rows_per_lane(NB); ++q) {
       const int i = lane + 64 * q;
-      xpre[q] = (t.slot < 0 && i < a.f) ? xr[i] : 0.0f;
+      xpre[q] = (t.slot < 0 && i < a.f) ? xrow[i] : 0.0f;
     }
   }
-  if (t.count > 0) gram_any<NB, BF>(acc, a, (int64_t)t.start, t.count, lds, lane);
+  f4 acc[num_tiles(NB)];  // set by the Gram
+  gram_any<NB, BF>(acc, a, (int64_t)t.start, t.count, lds, lane);
 
   if (t.slot >= 0) {  // partial Gram of a heavy row
     float* dst = a.partial + (int64_t)t.slot * num_tiles(NB) * 256;
@@ -71,14 +82,6 @@ __device__ __forceinline__ void run_task(const SweepArgs& a, const Task& t, floa
     for (int i = 0; i < num_tiles(NB); ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) dst[(i * 4 + r) * 64 + lane] = acc[i][r];
-    return;
-  }
-  float* xrow = a.x + (int64_t)t.row * a.ldf;
-  __bf16* xb = BF ? reinterpret_cast<__bf16*>(a.x_b) + (int64_t)t.row * a.ldb : nullptr;
-  if (t.count == 0) {  // no ratings: x_u = 0
-    for (int i = lane; i < a.ldf; i += 64) xrow[i] = 0.0f;
-    if (xb)
-      for (int i = lane; i < a.ldb; i += 64) xb[i] = (__bf16)0.0f;
     return;
   }
   add_ridge<NB>(acc, a.f, a.lambda * (float)t.count, lane);
