@@ -1,273 +1,355 @@
 // als_large.hip -- the half-sweep for 140 < f <= 256 (FP32 Gram + FP32
-// Cholesky), where the fused kernels' register-resident Gram tiles no
-// longer fit a wavefront.
+// Cholesky), where the augmented Gram (up to 153 tiles of 16 x 16) no
+// longer fits one wavefront's registers.
 //
-// One 256-thread work-group per row (reference semantics as the fused path:
-// als/src/cuda/als.cu:47-51, 462-466 and 905-909 for the Gram and ridge,
-// :1039-1054 for b, :935-997 for the solve):
-//   * Gram: chunks of 16 ratings are staged in LDS as rows of the augmented
-//     [y r] (width W = 8 ceil((f+1)/8), zero padded); each thread owns up to
-//     three 8x8 blocks (I <= J) of the augmented upper triangle and
-//     accumulates them in registers with FP32 fused multiply-adds.
-//   * The blocks are written to LDS as the packed upper triangle of the
-//     (f+1) x (f+1) augmented matrix (<= 33,153 floats, 130 KB at f = 256),
-//     the ridge lambda n_u is added to the diagonal, and a blocked
-//     right-looking Cholesky runs in place (panels of 8 pivots, then a
-//     rank-8 update of the trailing rows in 4x4 register tiles); the
-//     augmented column becomes z = U^{-T} b.
-//   * Back substitution U x = z in wavefront 0, z in registers.
-// Rows without ratings get x = 0; a non-positive pivot counts a failed row
-// and gives x = 0.
+// One 256-thread work-group (four wavefronts, one per SIMD) per row, same
+// arithmetic as the fused kernels (reference: als/src/cuda/als.cu:47-51,
+// 462-466 and 905-909 for the Gram and ridge, :1039-1054 for b, :935-997
+// for the solve):
+//   * Gram on MFMA (v_mfma_f32_16x16x4_f32): the NT = NB(NB+1)/2 upper
+//     tiles of the augmented [Y r]^T [Y r] are dealt round-robin to the
+//     four wavefronts (tile t to wavefront t % 4), which keep them in
+//     registers.  Chunks of 16 ratings are staged in LDS tile-major as in
+//     the fused kernels (tile I of a chunk gathered by wavefront I % 4 with
+//     global_load_lds_dwordx4; the rating column written by the wavefront
+//     that gathers the last tile), double-buffered with one barrier per
+//     chunk.
+//   * The tiles then go to LDS (1 KB each, in accumulator order), and a
+//     blocked right-looking Cholesky runs on them with three barriers per
+//     16-column block K: wavefront 0 factors the diagonal tile into
+//     W_K = U_KK^{-T} (the column-per-lane elimination of the fused
+//     kernels); the four wavefronts form the panel U_KJ = W_K A_KJ and the
+//     trailing update A_IJ -= U_KI^T U_KJ on MFMA, tile by tile.  The
+//     augmented column ends as z = U^{-T} b.
+//   * Blocked back substitution x_K = W_K^T (z_K - sum_{J>K} U_KJ x_J) in
+//     wavefront 0.
+// Rows without ratings get x = 0; a pivot that is not positive and finite
+// counts a failed row and gives x = 0.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "als_kernels.h"
+#include "als_device.h"
 
 namespace cumf {
 namespace {
 
-constexpr int kLT = 256;      // threads per row
-constexpr int kLC = 16;       // ratings per staged chunk
-constexpr int kMaxPairs = 3;  // 8x8 block pairs per thread: ceil(33*34/2 / 256)
+constexpr int kLT = 256;  // threads per row: four wavefronts
 constexpr int kLargeMaxF = 256;
 
-__host__ __device__ constexpr int packed_floats(int f) { return (f + 1) * (f + 2) / 2; }
-// Offset of row i in the packed upper triangle of an n x n matrix, minus i,
-// so that element (i, j >= i) is at row_base(i, n) + j.
-__device__ __forceinline__ int row_base(int i, int n) { return i * n - (i * (i - 1)) / 2 - i; }
+__host__ __device__ constexpr int qmax(int NB) { return (num_tiles(NB) + 3) / 4; }
 
+template <typename F, int... qs>
+__device__ __forceinline__ void static_for_impl(F&& fn, std::integer_sequence<int, qs...>) {
+  (fn(std::integral_constant<int, qs>{}), ...);
+}
+// fn(std::integral_constant<int, q>) for q = 0 .. N-1, unrolled.
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& fn) {
+  static_for_impl(fn, std::make_integer_sequence<int, N>{});
+}
+// LDS floats: the tiles, then the diagonal-tile scratch, the identity, the
+// last block's W, z and x.  The Gram's staging (2 chunks of NB tiles and
+// their ratings) overlays the tile area.
+__host__ __device__ constexpr int large_lds_floats(int NB) { return num_tiles(NB) * 256 + 3 * 256 + 32 * NB; }
+
+// Tile t of the upper triangle in tile_index order -> (I, J).
+__host__ __device__ constexpr int tile_row(int t, int NB) {
+  int I = 0;
+  while (t >= NB - I) {
+    t -= NB - I;
+    ++I;
+  }
+  return I;
+}
+__host__ __device__ constexpr int tile_col(int t, int NB) {
+  int I = 0;
+  while (t >= NB - I) {
+    t -= NB - I;
+    ++I;
+  }
+  return I + t;
+}
+
+// The Gram of wavefront W's tiles (t = W, W + 4, ...), left in acc, then
+// the ridge and the store of the tiles to their LDS slots.
+template <int NB, int W>
+__device__ __forceinline__ void gram_tiles(const SweepArgs& a, int64_t start, int count, float* lds, int lane) {
+  constexpr int NT = num_tiles(NB), Q = qmax(NB);
+  constexpr int CF = NB * 256;                // floats per staged chunk
+  constexpr bool WRITER = (NB - 1) % 4 == W;  // gathers the last tile, writes the ratings
+  const int g = lane >> 4, c = lane & 15;
+  const int sr = lane >> 2, sq = lane & 3;
+  const int ldf = (int)a.ldf;
+  const bool last_in = 16 * (NB - 1) + 4 * sq < ldf;
+  const int rcol = ldf - 16 * (NB - 1);  // the rating's column in the last tile
+  const int nch = (count + KB - 1) / KB;
+  f4 acc[Q];
+
+  auto load_ci = [&](int ch) -> int {
+    const int k = ch * KB + sr;
+    return (ch < nch && k < count) ? a.col[start + k] : -1;
+  };
+  auto load_r = [&](int ch) -> float {
+    const int k = ch * KB + lane;
+    return (lane < KB && ch < nch && k < count) ? a.val[start + k] : 0.0f;
+  };
+  auto issue = [&](int ch, int ci) {
+    float* yb = lds + (ch & 1) * CF;
+    const float* rp = ci >= 0 ? a.theta + (int64_t)ci * a.ldf + 4 * sq : kZeroRow;
+#pragma unroll
+    for (int I = W; I < NB - 1; I += 4)
+      __builtin_amdgcn_global_load_lds(rp + 16 * I, (lds_void_t*)(yb + I * 256), 16, 0, 0);
+    if constexpr (WRITER) {
+      // (kZeroRow holds 256 floats: a row past the range reads the last
+      // tile from its start)
+      const float* rl = (last_in && ci >= 0) ? rp + 16 * (NB - 1) : kZeroRow;
+      __builtin_amdgcn_global_load_lds(rl, (lds_void_t*)(yb + (NB - 1) * 256), 16, 0, 0);
+    }
+  };
+  auto mfma_chunk = [&](const float* yb, auto first) {
+#pragma unroll
+    for (int s = 0; s < KB / 4; ++s) {
+      const int row = 4 * s + g;
+      static_for<Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value, t = 4 * q + W;
+        if constexpr (t < NT) {
+          constexpr int I = tile_row(t, NB), J = tile_col(t, NB);
+          const float ai = yb[I * 256 + row * 16 + c], aj = yb[J * 256 + row * 16 + c];
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+              ai, aj, (decltype(first)::value && s == 0) ? f4{0.f, 0.f, 0.f, 0.f} : acc[q], 0, 0, 0);
+        }
+      });
+    }
+  };
+
+  float rv = WRITER ? load_r(0) : 0.0f;
+  issue(0, load_ci(0));
+  int ci_next = load_ci(1);
+  for (int ch = 0; ch < nch; ++ch) {
+    wait_vm_all();  // this wavefront's gathers of chunk ch (and its rating loads) are done
+    const float* yb = lds + (ch & 1) * CF;
+    if constexpr (WRITER) {
+      if (lane < KB) lds[(ch & 1) * CF + (NB - 1) * 256 + lane * 16 + rcol] = rv;
+    }
+    __syncthreads();  // chunk ch staged by all; chunk ch-1 consumed by all
+    if (ch + 1 < nch) issue(ch + 1, ci_next);
+    ci_next = load_ci(ch + 2);
+    if constexpr (WRITER) rv = load_r(ch + 1);
+    if (ch == 0) mfma_chunk(yb, std::true_type{});
+    else mfma_chunk(yb, std::false_type{});
+  }
+  wait_vm_all();
+  __syncthreads();  // every wavefront is done with the staging area
+
+  const float ridge = a.lambda * (float)count;
+  static_for<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value, t = 4 * q + W;
+    if constexpr (t < NT) {
+      constexpr int I = tile_row(t, NB), J = tile_col(t, NB);
+      if constexpr (I == J) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (c == 4 * g + r && 16 * I + c < a.f) acc[q][r] += ridge;
+      }
+      *reinterpret_cast<f4*>(lds + t * 256 + 4 * lane) = acc[q];
+    }
+  });
+}
+
+// Diagonal tile K (in accumulator order in its slot) -> W_K = U_KK^{-T},
+// column-major, into wdst; for the last block also U_KK back into the slot
+// (its augmented columns hold z).  Wavefront 0 only.
+template <int NB>
+__device__ __forceinline__ void diag_tile(float* slot, float* wdst, float* tsc, const float* ident, int live,
+                                          bool last, int lane) {
+  const int g = lane >> 4, c = lane & 15;
+  const f4 tv = *reinterpret_cast<const f4*>(slot + 4 * lane);
+  *reinterpret_cast<f4*>(tsc + 16 * c + 4 * g) = tv;  // column-major T
+  lds_order();
+  const bool trow = (g & 1) == 0;
+  const float* src = (trow ? tsc : ident) + 16 * c;
+  float av[16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const f4 v = *reinterpret_cast<const f4*>(src + 4 * q);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) av[4 * q + r] = v[r];
+  }
+  diag_pivots<true, false>(av, live, std::make_integer_sequence<int, 16>{});
+  if (last) {
+#pragma unroll
+    for (int i = 1; i < 16; ++i) av[i] = (trow && i > c) ? 0.0f : av[i];
+  }
+  lds_order();
+  float* dst = trow ? tsc + 16 * c : wdst + 16 * c;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    *reinterpret_cast<f4*>(dst + 4 * q) = f4{av[4 * q], av[4 * q + 1], av[4 * q + 2], av[4 * q + 3]};
+  lds_order();
+  if (last) *reinterpret_cast<f4*>(slot + 4 * lane) = *reinterpret_cast<const f4*>(tsc + 16 * c + 4 * g);
+}
+
+template <int NB>
 __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void als_large_kernel(SweepArgs a) {
   extern __shared__ float lds[];
-  const int t = threadIdx.x;
+  constexpr int NT = num_tiles(NB);
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
   const int64_t u = blockIdx.x;
-  const int f = a.f, F1 = f + 1, FB = (F1 + 7) / 8, W = 8 * FB;
+  const int f = a.f;
   const int64_t s0 = a.row_ptr[u];
   const int cnt = a.row_ptr[u + 1] - a.row_ptr[u];
   float* xrow = a.x + u * a.ldf;
   if (cnt <= 0) {
-    for (int i = t; i < (int)a.ldf; i += kLT) xrow[i] = 0.0f;
+    for (int i = threadIdx.x; i < (int)a.ldf; i += kLT) xrow[i] = 0.0f;
     return;
   }
-
-  // This thread's block pairs (I, J), I <= J < FB, numbered row by row.
-  const int nbp = FB * (FB + 1) / 2;
-  int pI[kMaxPairs], pJ[kMaxPairs];
-#pragma unroll
-  for (int q = 0; q < kMaxPairs; ++q) {
-    int p = t + q * kLT;
-    pI[q] = -1;
-    pJ[q] = -1;
-    if (p < nbp) {
-      int I = 0;
-      while (p >= FB - I) {
-        p -= FB - I;
-        ++I;
-      }
-      pI[q] = I;
-      pJ[q] = I + p;
-    }
+  float* tsc = lds + NT * 256;
+  float* ident = tsc + 256;
+  float* wlast = ident + 256;
+  float* zb = wlast + 256;
+  float* xs = zb + 16 * NB;
+  if (wv == 0) {
+    const f4 e = {4 * g == c ? 1.0f : 0.0f, 4 * g + 1 == c ? 1.0f : 0.0f, 4 * g + 2 == c ? 1.0f : 0.0f,
+                  4 * g + 3 == c ? 1.0f : 0.0f};
+    *reinterpret_cast<f4*>(ident + 16 * c + 4 * g) = e;
   }
-  float acc[kMaxPairs][8][8];
-#pragma unroll
-  for (int q = 0; q < kMaxPairs; ++q)
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-#pragma unroll
-      for (int s = 0; s < 8; ++s) acc[q][r][s] = 0.0f;
 
-  float* Y = lds;  // [kLC][W]
-  for (int c0 = 0; c0 < cnt; c0 += kLC) {
-    const int nk = min(kLC, cnt - c0);
-    __syncthreads();  // the previous chunk has been consumed
-    for (int e = t; e < kLC * W; e += kLT) {
-      const int k = e / W, i = e - k * W;
-      float v = 0.0f;
-      if (k < nk) {
-        if (i < f) v = a.theta[(int64_t)a.col[s0 + c0 + k] * a.ldf + i];
-        else if (i == f) v = a.val[s0 + c0 + k];
-      }
-      Y[e] = v;
-    }
-    __syncthreads();
-    for (int k = 0; k < nk; ++k) {
-      const float* yk = Y + k * W;
-#pragma unroll
-      for (int q = 0; q < kMaxPairs; ++q) {
-        if (pI[q] < 0) continue;
-        const float4 a0 = *reinterpret_cast<const float4*>(yk + 8 * pI[q]);
-        const float4 a1 = *reinterpret_cast<const float4*>(yk + 8 * pI[q] + 4);
-        const float4 b0 = *reinterpret_cast<const float4*>(yk + 8 * pJ[q]);
-        const float4 b1 = *reinterpret_cast<const float4*>(yk + 8 * pJ[q] + 4);
-        const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-        const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-#pragma unroll
-          for (int s = 0; s < 8; ++s) acc[q][r][s] = fmaf(av[r], bv[s], acc[q][r][s]);
-      }
-    }
+  switch (wv) {
+    case 0: gram_tiles<NB, 0>(a, s0, cnt, lds, lane); break;
+    case 1: gram_tiles<NB, 1>(a, s0, cnt, lds, lane); break;
+    case 2: gram_tiles<NB, 2>(a, s0, cnt, lds, lane); break;
+    default: gram_tiles<NB, 3>(a, s0, cnt, lds, lane); break;
   }
-  __syncthreads();  // staging done; the packed matrix reuses the space
+  __syncthreads();  // all tiles in their slots
 
-  float* P = lds;  // packed upper triangle, F1 x F1
+  auto slot = [&](int I, int J) { return lds + tile_index(I, J, NB) * 256; };
+  for (int K = 0; K < NB; ++K) {
+    if (16 * K >= f) break;  // only the last block can hold no pivot of the f x f system
+    const bool last = K + 1 == NB;
+    if (wv == 0) diag_tile<NB>(slot(K, K), last ? wlast : slot(K, K), tsc, ident, f - 16 * K, last, lane);
+    __syncthreads();  // W_K is in LDS
+    if (last) break;
+    {
+      // Panel: U_KJ = W_K A_KJ; A operand W_K[c][4g+s] from the column-major W.
+      const float* w = slot(K, K);
+      f4 wa;
 #pragma unroll
-  for (int q = 0; q < kMaxPairs; ++q) {
-    if (pI[q] < 0) continue;
+      for (int s = 0; s < 4; ++s) wa[s] = w[16 * (4 * g + s) + c];
+      for (int J = K + 1 + wv; J < NB; J += 4) {
+        float* p = slot(K, J);
+        const f4 b = *reinterpret_cast<const f4*>(p + 4 * lane);
+        f4 d = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int i = 8 * pI[q] + r;
-      if (i >= F1) continue;
-      const int rb = row_base(i, F1);
-#pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        const int j = 8 * pJ[q] + s;
-        if (j >= i && j < F1) P[rb + j] = acc[q][r][s];
+        for (int s = 0; s < 4; ++s) d = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[s], b[s], d, 0, 0, 0);
+        *reinterpret_cast<f4*>(p + 4 * lane) = d;
       }
     }
-  }
-  __syncthreads();
-  const float ridge = a.lambda * (float)cnt;
-  for (int i = t; i < f; i += kLT) P[row_base(i, F1) + i] += ridge;
-
-  // Blocked right-looking Cholesky.  Panels of 8 pivots are factored row
-  // by row, one thread per column: the pivot and the panel's part of row k
-  // are read, a barrier, then each thread scales its element of row k and
-  // updates the panel rows below it in its column.  The rows past the panel
-  // then take one rank-8 update in 4x4 register tiles (a 16 x 16 thread
-  // grid over the tiles), eight times fewer LDS accesses than updating
-  // after every pivot.  The augmented column f rides along, so it ends as
-  // z = U^{-T} b.
-  const int ta = t >> 4, tc = t & 15;
-  bool ok = true;
-  for (int kb = 0; kb < f; kb += 8) {
-    const int ke = min(kb + 8, f);
-    for (int k = kb; k < ke; ++k) {
-      __syncthreads();
-      const int rbk = row_base(k, F1);
-      const float d = P[rbk + k];
-      if (!(d > 0.0f && d < __builtin_huge_valf())) {  // same value in every thread
-        ok = false;
-        break;
-      }
-      const float inv = rsqrtf(d);
-      float uk[7];
+    __syncthreads();  // the panel is final
+    {
+      // Trailing update of tiles (I, J), K < I <= J, dealt round-robin.
+      int q = 0;
+      for (int I = K + 1; I < NB; ++I) {
+        const f4 uv = *reinterpret_cast<const f4*>(slot(K, I) + 4 * lane);
+        const f4 nu = {-uv[0], -uv[1], -uv[2], -uv[3]};
+        for (int J = I; J < NB; ++J, ++q) {
+          if ((q & 3) != wv) continue;
+          float* p = slot(I, J);
+          const f4 v = *reinterpret_cast<const f4*>(slot(K, J) + 4 * lane);
+          f4 d = *reinterpret_cast<const f4*>(p + 4 * lane);
 #pragma unroll
-      for (int q = 0; q < 7; ++q) {
-        const int i = k + 1 + q;
-        uk[q] = i < ke ? P[rbk + i] * inv : 0.0f;
-      }
-      __syncthreads();
-      for (int j = k + t; j < F1; j += kLT) {
-        const float v = P[rbk + j] * inv;
-        P[rbk + j] = v;
-#pragma unroll
-        for (int q = 0; q < 7; ++q) {
-          const int i = k + 1 + q;
-          if (i < ke && i <= j) {
-            const int ri = row_base(i, F1);
-            P[ri + j] = fmaf(-uk[q], v, P[ri + j]);
-          }
+          for (int s = 0; s < 4; ++s) d = __builtin_amdgcn_mfma_f32_16x16x4f32(nu[s], v[s], d, 0, 0, 0);
+          *reinterpret_cast<f4*>(p + 4 * lane) = d;
         }
       }
     }
-    if (!ok || ke >= f) break;  // uniform
-    __syncthreads();            // the panel rows are final
-    const int T = (F1 - ke + 3) >> 2;
-    for (int ti = ta; ti < T; ti += 16) {
-      const int i0 = ke + 4 * ti;
-      if (i0 >= f) break;
-      for (int tj = ti + ((tc - ti) & 15); tj < T; tj += 16) {  // tj >= ti, tj = tc (mod 16)
-        const int j0 = ke + 4 * tj;
-        float acc4[4][4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) acc4[r][c] = 0.0f;
-        for (int k = kb; k < ke; ++k) {
-          const int rbk = row_base(k, F1);
-          float ui[4], uj[4];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            ui[r] = P[rbk + min(i0 + r, F1 - 1)];  // clamped reads; their results are not stored
-            uj[r] = P[rbk + min(j0 + r, F1 - 1)];
-          }
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) acc4[r][c] = fmaf(ui[r], uj[c], acc4[r][c]);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = i0 + r;
-          if (i >= f) continue;
-          const int ri = row_base(i, F1);
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const int j = j0 + c;
-            if (j >= i && j < F1) P[ri + j] -= acc4[r][c];
-          }
-        }
-      }
-    }
+    __syncthreads();  // the trailing tiles are updated
   }
-  __syncthreads();
-  if (!ok) {
-    if (t == 0) atomicAdd(a.fail_counter, 1);
-    for (int i = t; i < (int)a.ldf; i += kLT) xrow[i] = 0.0f;
-    return;
-  }
-  if (t >= 64) return;
+  if (wv != 0) return;
 
-  // Back substitution in wavefront 0: entry i of z and x in lane i % 64,
-  // slot i / 64.
-  const int lane = t;
-  float z[kLargeMaxF / 64], xv[kLargeMaxF / 64];
-#pragma unroll
-  for (int q = 0; q < kLargeMaxF / 64; ++q) {
-    const int i = lane + 64 * q;
-    z[q] = i < f ? P[row_base(i, F1) + f] : 0.0f;
-    xv[q] = 0.0f;
+  // z out of the augmented column (column ldf of the system, in block NB-1).
+  const int cb = (int)a.ldf - 16 * (NB - 1);
+  float* sink = tsc + 4 * lane;
+  for (int I = 0; I < NB; ++I) {
+    const f4 v = *reinterpret_cast<const f4*>(slot(I, NB - 1) + 4 * lane);
+    *reinterpret_cast<f4*>(c == cb ? zb + 16 * I + 4 * g : sink) = v;
   }
-  for (int k = f - 1; k >= 0; --k) {
-    const int qk = k >> 6, lk = k & 63;
-    float zk_src = z[0];
+  lds_order();
+  bool fin = true;
+  for (int K = NB - 1; K >= 0; --K) {
+    const float* w = (K + 1 == NB ? wlast : slot(K, K)) + 16 * c;
+    const f4 w0 = *reinterpret_cast<const f4*>(w), w1 = *reinterpret_cast<const f4*>(w + 4);
+    const f4 w2 = *reinterpret_cast<const f4*>(w + 8), w3 = *reinterpret_cast<const f4*>(w + 12);
+    const float zv = zb[16 * K + 4 * g + (c >> 2)];
+    float y[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int J = K + 1; J < NB; ++J) {
+      const f4 uv = *reinterpret_cast<const f4*>(slot(K, J) + 4 * lane);
+      const float xj = xs[16 * J + c];
 #pragma unroll
-    for (int q = 1; q < kLargeMaxF / 64; ++q)
-      if (qk == q) zk_src = z[q];
-    const float zk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zk_src), lk));
-    const float xk = zk / P[row_base(k, F1) + k];
-#pragma unroll
-    for (int q = 0; q < kLargeMaxF / 64; ++q) {
-      const int i = lane + 64 * q;
-      if (i == k) xv[q] = xk;
-      if (i < k) z[q] = fmaf(-P[row_base(i, F1) + k], xk, z[q]);
+      for (int r = 0; r < 4; ++r) y[r] = fmaf(uv[r], xj, y[r]);
     }
-  }
+    // t_i for i = 4g + (c >> 2), so t_j sits in lane 4j.
+    const float t = zv - row_reduce_scatter(y[0], y[1], y[2], y[3], c);
+    const f4 wq[4] = {w0, w1, w2, w3};
+    float x = 0.0f;
 #pragma unroll
-  for (int q = 0; q < kLargeMaxF / 64; ++q) {
-    const int i = lane + 64 * q;
-    if (i < (int)a.ldf) xrow[i] = i < f ? xv[q] : 0.0f;
+    for (int j = 0; j < 16; ++j) x = fmaf(wq[j >> 2][j & 3], readlane_f(t, 4 * j), x);
+    x = 16 * K + c < f ? x : 0.0f;  // a block past f has no W_K and gives x_K = 0
+    fin = fin && fabsf(x) < __builtin_huge_valf();
+    if (g == 0) xs[16 * K + c] = x;
+    lds_order();
   }
+  // A pivot that was not positive and finite leaves NaN or infinities in x.
+  const bool ok = !__any(!fin);
+  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
+  for (int i = lane; i < (int)a.ldf; i += 64) xrow[i] = (ok && i < f) ? xs[i] : 0.0f;
+}
+
+template <int NB>
+hipError_t set_lds_limit() {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&als_large_kernel<NB>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(sizeof(float) * large_lds_floats(NB)));
 }
 
 }  // namespace
 
 size_t large_lds_bytes(int f) {
-  const int W = 8 * ((f + 8) / 8);  // staged row width, 8 ceil((f+1)/8)
-  const int staged = kLC * W, packed = packed_floats(f);
-  return sizeof(float) * (size_t)(packed > staged ? packed : staged);
+  const int NB = sweep_nb(f);
+  return sizeof(float) * (size_t)large_lds_floats(NB);
 }
 
 hipError_t prepare_large_kernel(int f) {
   if (f <= 0 || f > kLargeMaxF) return hipErrorInvalidValue;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&als_large_kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)large_lds_bytes(f));
+  switch (sweep_nb(f)) {
+    case 10: return set_lds_limit<10>();
+    case 11: return set_lds_limit<11>();
+    case 12: return set_lds_limit<12>();
+    case 13: return set_lds_limit<13>();
+    case 14: return set_lds_limit<14>();
+    case 15: return set_lds_limit<15>();
+    case 16: return set_lds_limit<16>();
+    case 17: return set_lds_limit<17>();
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_large_kernel(const SweepArgs& a, int64_t rows, hipStream_t s) {
   if (a.f <= 0 || a.f > kLargeMaxF || rows < 0) return hipErrorInvalidValue;
   if (rows == 0) return hipSuccess;
-  hipLaunchKernelGGL(als_large_kernel, dim3((unsigned)rows), dim3(kLT), large_lds_bytes(a.f), s, a);
+  const dim3 grid((unsigned)rows), block(kLT);
+  const size_t shmem = large_lds_bytes(a.f);
+  switch (sweep_nb(a.f)) {
+    case 10: hipLaunchKernelGGL(als_large_kernel<10>, grid, block, shmem, s, a); break;
+    case 11: hipLaunchKernelGGL(als_large_kernel<11>, grid, block, shmem, s, a); break;
+    case 12: hipLaunchKernelGGL(als_large_kernel<12>, grid, block, shmem, s, a); break;
+    case 13: hipLaunchKernelGGL(als_large_kernel<13>, grid, block, shmem, s, a); break;
+    case 14: hipLaunchKernelGGL(als_large_kernel<14>, grid, block, shmem, s, a); break;
+    case 15: hipLaunchKernelGGL(als_large_kernel<15>, grid, block, shmem, s, a); break;
+    case 16: hipLaunchKernelGGL(als_large_kernel<16>, grid, block, shmem, s, a); break;
+    case 17: hipLaunchKernelGGL(als_large_kernel<17>, grid, block, shmem, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
