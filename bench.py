@@ -107,6 +107,16 @@ def cpu_baseline(ds, f, lam, theta, X, target_s):
             "ratings_per_s": ds.nnz / total}
 
 
+# What the committed counter passes show limits the dominant kernel, per
+# profiled configuration (profiles/als_task_kernel_netflix480k_f100_counters.md).
+LIMITER = {
+    ("netflix_480k_17770_99m", "bf16-cg-tuned", 100):
+        "X launch: issue and latency of the per-row Cholesky (matrix pipe 43% busy, 44% of wave time "
+        "waiting to issue, ~3,000 VALU instructions per row); Theta launch: BF16 gathers beyond L2 "
+        "(44% of wave time waiting on memory)",
+}
+
+
 def measured_traffic(workload, path, f):
     """HBM bytes per launch of the fused kernel for this workload and path,
     from the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE,
@@ -440,7 +450,8 @@ def main():
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "avg_launch_ms": task_ms,
-                     "flops_per_launch": gram_flops_launch},
+                     "flops_per_launch": gram_flops_launch,
+                     "limiter": LIMITER.get((args.workload, traffic_key, f))},
         "gather_roofline": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
                             "bytes_per_launch": gather_bytes_launch,
                             "achieved": gather_bytes_launch / (task_ms * 1e-3) / 1e9,

@@ -14,11 +14,15 @@
 //     NB(NB+1)/2 upper 16x16 tiles (FP32: v_mfma_f32_16x16x4_f32; BF16:
 //     v_mfma_f32_16x16x32_bf16 with the rating split into a BF16 high and
 //     low column).  The extra column(s) carry b = Y^T r.
-//   * The accumulators stay in registers; their layout (lane l holds rows
-//     16I + 4(l/16) + {0..3} of column 16J + l%16) is a 2-D block-cyclic
-//     distribution on which a right-looking Cholesky runs in place, leaving
-//     z = U^{-T} b in the augmented column; U and z go to LDS for the back
-//     substitution.
+//   * The accumulators stay in registers (lane l holds rows
+//     16I + 4(l/16) + {0..3} of column 16J + l%16 of tile (I, J)), and a
+//     blocked right-looking Cholesky runs on the tiles in place: per
+//     16-column block K the diagonal tile is factored on the VALU into
+//     W_K = U_KK^{-T} (column-per-lane elimination with DPP broadcasts),
+//     then the panel U_KJ = W_K A_KJ and the trailing update
+//     A_IJ -= U_KI^T U_KJ run on MFMA, leaving z = U^{-T} b in the
+//     augmented column; a blocked back substitution with the W_K (in LDS)
+//     gives x.
 //   * Or the mixed-precision solve (cg_solve): FP16 CG on the Jacobi-scaled
 //     matrix in LDS inside FP32 refinement with FP64 acceptance residuals.
 //   * Rows with more than `split` ratings are cut into chunks whose partial
