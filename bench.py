@@ -202,6 +202,37 @@ def train_network(G, ds, f, lam, args, prec):
     return net, learned
 
 
+def learned_report(G, ds, theta0, f, lam, args, prec, fp64_rmse):
+    """The learned warm-start form at its configured setting: CG capped at 3
+    steps in one round, Cholesky whenever the relative residual stays above
+    1e-5.  A 10-iteration training from the seeded start with the network's
+    prediction as the CG start, and the same with the previous factor as the
+    start (the form without the network); untimed network training first."""
+    net, info = train_network(G, ds, f, lam, args, prec)
+    out = {"setting": "cg_iters 3, cg_rounds 1, cg_tol 1e-5, Cholesky fallback", "network": info}
+    for name, warm in (("with_network", G.CUMF_WARM_MLP), ("without_network_prev_start", G.CUMF_WARM_PREV)):
+        h = G.ALS(f, lam, device=0, precision=prec, solver=G.CUMF_SOLVER_CG, cg_iters=3, cg_rounds=1,
+                  cg_tol=1e-5, warm_start=warm)
+        if warm == G.CUMF_WARM_MLP:
+            h.set_mlp(net)
+        h.set_dataset(ds)
+        h.set_factors(theta=theta0)
+        h.synchronize()
+        t0 = time.perf_counter()
+        h.iterate(TRAIN_ITERS)
+        h.synchronize()
+        t = time.perf_counter() - t0
+        rm = h.rmse()
+        fb = h.stats()["cg_fallback_rows"]
+        h.close()
+        out[name] = {"train_s": t, "heldout_rmse": rm,
+                     "rmse_delta_vs_fp64_oracle": (rm - fp64_rmse) if fp64_rmse else None,
+                     "cg_fallback_rows": fb, "fallback_rate": fb / (TRAIN_ITERS * (ds.m + ds.n))}
+    net.close()
+    log(f"learned form: {out}")
+    return out
+
+
 def run_path(G, ds, theta0, f, lam, args, name, opts, tune, net=None, report_rows=0):
     """Training, steady-state timing and per-kernel split of one path."""
     h = G.ALS(f, lam, device=0, **opts)
@@ -332,6 +363,8 @@ def main():
     ap.add_argument("--tile-table", default="", help="load degree-bucket launch shapes from this file")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--no-dropin", action="store_true", help="skip timing doALSWithCSR")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned warm-start form at its configured setting (3 CG steps, tol 1e-5)")
     ap.add_argument("--precision-report", type=int, default=256, metavar="ROWS",
                     help="FP64 residual and factor error of one X half-sweep on ROWS sampled rows (0: off)")
     ap.add_argument("--cpu-seconds", type=float, default=24.0)
@@ -377,6 +410,8 @@ def main():
     st, kst, opt = res["stats"], res["kernel_stats"], res["opt"]
     if learned is not None:
         learned["cg_fallback_rows_training"] = res["train_fallback"]
+    elif not args.no_learned and f <= 140:
+        learned = learned_report(G, ds, theta0, f, lam, args, prec, fp64_rmse)
     is_plain = prec == G.CUMF_PREC_FP32 and solver == G.CUMF_SOLVER_CHOLESKY
     plain = None
     if not is_plain and not args.no_plain:

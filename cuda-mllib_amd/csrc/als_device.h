@@ -621,14 +621,22 @@ __device__ __forceinline__ void diag_pivot(float (&a)[16], int live) {
     }
   }
 }
-template <bool PARTIAL, bool GUARD, int... ks>
-__device__ __forceinline__ void diag_pivots(float (&a)[16], int live, std::integer_sequence<int, ks...>) {
-  (diag_pivot<ks, PARTIAL, GUARD>(a, live), ...);
+// The 16 pivots; after pivot k, hook(integral_constant<k>) issues work
+// that does not depend on the elimination (see chol_step).
+struct NoHook {
+  template <typename T>
+  __device__ __forceinline__ void operator()(T) const {}
+};
+template <bool PARTIAL, bool GUARD, typename H, int... ks>
+__device__ __forceinline__ void diag_pivots(float (&a)[16], int live, const H& hook,
+                                            std::integer_sequence<int, ks...>) {
+  ((diag_pivot<ks, PARTIAL, GUARD>(a, live), hook(std::integral_constant<int, ks>{})), ...);
 }
 
-template <int NB, int K>
+template <int NB, int K, typename H>
 __device__ __forceinline__ void chol_diag(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ wcm,
-                                          float* __restrict__ tsc, const float* __restrict__ ident, int lane) {
+                                          float* __restrict__ tsc, const float* __restrict__ ident, int lane,
+                                          const H& hook) {
   const int g = lane >> 4, c = lane & 15;
   constexpr int TK = tile_index(K, K, NB);
   // Column-major copy of T: tsc[16 c + i] = T[i][c].
@@ -648,7 +656,7 @@ __device__ __forceinline__ void chol_diag(f4 (&acc)[num_tiles(NB)], int f, float
   }
   // Only the last two blocks can hold pivots at or past f (sweep_nb gives
   // 16 (NB - 1) <= f + 6, so f - 16 K >= 26 for K <= NB - 3).
-  diag_pivots<K + 2 >= NB, K + 1 == NB>(a, f - 16 * K, std::make_integer_sequence<int, 16>{});
+  diag_pivots<K + 2 >= NB, K + 1 == NB>(a, f - 16 * K, hook, std::make_integer_sequence<int, 16>{});
   if constexpr (K + 1 == NB) {
     // The last block's U_KK goes back to the accumulators (its augmented
     // columns hold z), upper triangular: what elimination leaves below the
@@ -684,32 +692,80 @@ __device__ __forceinline__ void chol_panel(f4 (&acc)[num_tiles(NB)], const float
   }
 }
 
+// Trailing update of block K, A_IJ -= U_KI^T U_KJ for K < I <= J, as a
+// list of MFMAs: tile (K+1, K+1) first (the next diagonal factorisation
+// waits for it), then the rest of row K+1 (the next panel reads it), then
+// rows K+2 ..; four MFMAs per tile, m = 4 u + s for the u-th tile.
+__host__ __device__ constexpr int trailing_tiles(int NB, int K) { return (NB - K - 1) * (NB - K) / 2; }
+__host__ __device__ constexpr int trailing_tile_row(int NB, int K, int u) {
+  int I = K + 1;
+  while (u >= NB - I) {
+    u -= NB - I;
+    ++I;
+  }
+  return I;
+}
+__host__ __device__ constexpr int trailing_tile_col(int NB, int K, int u) {
+  int I = K + 1;
+  while (u >= NB - I) {
+    u -= NB - I;
+    ++I;
+  }
+  return I + u;
+}
+template <int NB, int K, int m>
+__device__ __forceinline__ void trailing_mfma(f4 (&acc)[num_tiles(NB)]) {
+  constexpr int u = m / 4, s = m % 4;
+  constexpr int I = trailing_tile_row(NB, K, u), J = trailing_tile_col(NB, K, u);
+  constexpr int tij = tile_index(I, J, NB), tki = tile_index(K, I, NB), tkj = tile_index(K, J, NB);
+  acc[tij] = __builtin_amdgcn_mfma_f32_16x16x4f32(-acc[tki][s], acc[tkj][s], acc[tij], 0, 0, 0);
+}
+template <int NB, int K, int lo, int... ms>
+__device__ __forceinline__ void trailing_mfmas_seq(f4 (&acc)[num_tiles(NB)], std::integer_sequence<int, ms...>) {
+  (trailing_mfma<NB, K, lo + ms>(acc), ...);
+}
+// MFMAs lo .. hi-1 of block K's trailing update.
+template <int NB, int K, int lo, int hi>
+__device__ __forceinline__ void trailing_mfmas(f4 (&acc)[num_tiles(NB)]) {
+  if constexpr (hi > lo) trailing_mfmas_seq<NB, K, lo>(acc, std::make_integer_sequence<int, hi - lo>{});
+}
+
+// Block K's diagonal tile is factored (W_K in LDS): its panel, then its
+// trailing update and the next block, with one block of lookahead.  Tile
+// (K+1, K+1) is updated first; of the rest, up to 32 MFMAs are issued
+// between the 16 pivots of block K+1's diagonal factorisation (two after
+// each), where the matrix pipe runs them beside the elimination's VALU
+// work, and the remainder just before it.  Nothing the elimination reads
+// is written by them: it reads only tile (K+1, K+1) and LDS.
 template <int NB, int K>
-__device__ __forceinline__ void chol_trailing(f4 (&acc)[num_tiles(NB)]) {
-#pragma unroll
-  for (int I = K + 1; I < NB; ++I) {
-    const f4 u = acc[tile_index(K, I, NB)];
-    const f4 nu = {-u[0], -u[1], -u[2], -u[3]};
-#pragma unroll
-    for (int J = I; J < NB; ++J)
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-        acc[tile_index(I, J, NB)] =
-            __builtin_amdgcn_mfma_f32_16x16x4f32(nu[s], acc[tile_index(K, J, NB)][s], acc[tile_index(I, J, NB)], 0, 0, 0);
+__device__ __forceinline__ void chol_step(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ wcm,
+                                          float* __restrict__ tsc, const float* __restrict__ ident, int lane) {
+  if constexpr (K + 1 < NB) {
+    chol_panel<NB, K>(acc, wcm, lane);
+    constexpr int M = 4 * trailing_tiles(NB, K);  // MFMAs of the whole trailing update
+    constexpr int R = M - 4 < 32 ? M - 4 : 32;     // interleaved with the next diagonal tile
+    constexpr int P = M - R;                       // issued before it (tile (K+1, K+1) included)
+    trailing_mfmas<NB, K, 0, P>(acc);
+    // The last block may hold only the right-hand side column(s); then
+    // nothing follows (R = 0).
+    if (K + 2 < NB || 16 * (K + 1) < f) {
+      const auto hook = [&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        __builtin_amdgcn_sched_barrier(0);
+        trailing_mfmas<NB, K, P + k * R / 16, P + (k + 1) * R / 16>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+      };
+      chol_diag<NB, K + 1>(acc, f, wcm, tsc, ident, lane, hook);
+    }
+    chol_step<NB, K + 1>(acc, f, wcm, tsc, ident, lane);
   }
 }
 
-template <int NB, int K>
+template <int NB>
 __device__ __forceinline__ void chol_blocks(f4 (&acc)[num_tiles(NB)], int f, float* __restrict__ wcm,
                                             float* __restrict__ tsc, const float* __restrict__ ident, int lane) {
-  // Every block but the last runs unconditionally (one basic block for the
-  // scheduler); the last may hold only the right-hand side column(s).
-  if (K + 1 < NB || 16 * K < f) {
-    chol_diag<NB, K>(acc, f, wcm, tsc, ident, lane);
-    chol_panel<NB, K>(acc, wcm, lane);
-    chol_trailing<NB, K>(acc);
-  }
-  if constexpr (K + 1 < NB) chol_blocks<NB, K + 1>(acc, f, wcm, tsc, ident, lane);
+  chol_diag<NB, 0>(acc, f, wcm, tsc, ident, lane, NoHook{});
+  chol_step<NB, 0>(acc, f, wcm, tsc, ident, lane);
 }
 
 
@@ -761,7 +817,7 @@ __device__ __forceinline__ bool chol_solve(f4 (&acc)[num_tiles(NB)], int f, int 
                   4 * g + 3 == c ? 1.0f : 0.0f};
     *reinterpret_cast<f4*>(ident + 16 * c + 4 * g) = e;
   }
-  chol_blocks<NB, 0>(acc, f, wcm, tsc, ident, lane);
+  chol_blocks<NB>(acc, f, wcm, tsc, ident, lane);
 
   // z (and the second column's z) out of the augmented column(s), which
   // sweep_nb places in the last column block (bcol, bcol2 >= 16 (NB - 1)).

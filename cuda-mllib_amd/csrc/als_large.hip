@@ -175,7 +175,7 @@ __device__ __forceinline__ void diag_tile(float* slot, float* wdst, float* tsc, 
 #pragma unroll
     for (int r = 0; r < 4; ++r) av[4 * q + r] = v[r];
   }
-  diag_pivots<true, false>(av, live, std::make_integer_sequence<int, 16>{});
+  diag_pivots<true, false>(av, live, NoHook{}, std::make_integer_sequence<int, 16>{});
   if (last) {
 #pragma unroll
     for (int i = 1; i < 16; ++i) av[i] = (trow && i > c) ? 0.0f : av[i];
