@@ -59,6 +59,7 @@ struct BucketCfg {
   int variant = 2;
   int persistent = 0;
   int solver = -1;
+  int occupancy = 0;  // at most this many wavefronts per CU (0: as registers and LDS allow)
 };
 
 // Host half of the planner: every row with at most `split` ratings is one
@@ -287,6 +288,7 @@ hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::Sw
   x.tasks = a.tasks + p.bucket_t0[b];
   x.ntasks = p.bucket_t1[b] - p.bucket_t0[b];
   x.persistent = cfg.persistent;
+  x.occupancy = cfg.persistent ? 0 : cfg.occupancy;
   x.task_counter = h->counters.as<int>() + side * kBuckets + b;
   if (cfg.solver >= 0) x.solver = cfg.solver;
   return task_launch(x, cfg.variant, h->stream);
@@ -299,7 +301,7 @@ int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs&
   if (!h->tiled) return timed(h, Timer::TASK, [&] { return task_launch(a, default_variant(h->f), h->stream); });
   // A side whose non-empty buckets share one shape and no persistent grid is
   // one launch (launches in sequence drain the GPU between buckets).
-  int v0 = 0, s0 = -2;
+  int v0 = 0, s0 = -2, o0 = 0;
   bool uniform = true;
   for (int b = 0; b < kBuckets; ++b) {
     if (p.bucket_t1[b] <= p.bucket_t0[b]) continue;
@@ -308,12 +310,14 @@ int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs&
     if (v0 == 0) {
       v0 = c.variant;
       s0 = sv;
+      o0 = c.occupancy;
     }
-    uniform = uniform && c.persistent == 0 && c.variant == v0 && sv == s0;
+    uniform = uniform && c.persistent == 0 && c.variant == v0 && sv == s0 && c.occupancy == o0;
   }
   if (uniform && v0 != 0) {
     cumf::SweepArgs x = a;
     x.solver = s0;
+    x.occupancy = o0;
     return timed(h, Timer::TASK, [&] { return task_launch(x, v0, h->stream); });
   }
   bool persistent = false;
@@ -1097,9 +1101,10 @@ int autotune_side(cumf_handle* h, int side, int reps, const TuneStart& start) {
   }
   h->tile_split[side] = best_split;
   CUMF_TRY(rebuild_plans(h));
-  // 2. Launch shape per degree bucket: register budget x persistent grid,
-  //    and on a CG handle the solver (a CG bucket is timed with the
-  //    Cholesky re-solve of the rows it hands back).
+  // 2. Launch shape per degree bucket: register budget x (one task per
+  //    group | persistent grid | occupancy cap), and on a CG handle the
+  //    solver (a CG bucket is timed with the Cholesky re-solve of the rows
+  //    it hands back).
   SideBufs b = side_bufs(h, side);
   cumf::SweepArgs a{};
   CUMF_TRY(fill_args(h, *b.p, *b.rowp, *b.col, *b.val, b.theta, b.x, *b.theta_b, *b.x_b, nullptr, a));
@@ -1113,12 +1118,17 @@ int autotune_side(cumf_handle* h, int side, int reps, const TuneStart& start) {
     BucketCfg bc;
     for (int sol : solvers)
       for (int variant : {2, 1})
-        for (int pers : {0, h->num_cus * (variant == 1 ? 4 : 8)}) {
-          BucketCfg cfg{variant, pers, sol};
+        for (int pers : {0, h->num_cus * (variant == 1 ? 4 : 8), -7, -6, -3}) {
+          // pers < 0: a one-task-per-group launch capped at -pers wavefronts
+          // per CU (7 or 6 of the eight a two-wavefront budget allows, 3 of
+          // the four of a one-wavefront budget)
+          if (pers < 0 && (variant == 1) != (pers == -3)) continue;
+          BucketCfg cfg{variant, std::max(pers, 0), sol, std::max(-pers, 0)};
           double ms = 0;
           CUMF_TRY(time_min_ms(h, reps, restore, [&] {
             if (cg) CUMF_HIP_TRY(cumf::launch_fallback_reset(a.fb_count, 1, h->stream));
-            if (pers) CUMF_HIP_TRY(cumf::launch_fallback_reset(h->counters.as<int>() + side * kBuckets + k, 1, h->stream));
+            if (cfg.persistent)
+              CUMF_HIP_TRY(cumf::launch_fallback_reset(h->counters.as<int>() + side * kBuckets + k, 1, h->stream));
             CUMF_HIP_TRY(bucket_launch(h, side, *b.p, a, k, cfg));
             if (cg) CUMF_HIP_TRY(fallback_launch(a, dv, h->stream));
             return CUMF_OK;
@@ -1222,15 +1232,16 @@ int cumf_save_tile_table(cumf_handle* h, const char* path) {
   std::fprintf(fp, "# cumf tile table: f %d precision %d solver %d\n", h->f, h->opt.precision, h->opt.solver);
   std::fprintf(fp, "# side <0 X | 1 Theta> split <ratings per task> finish <1|2> fallback <1|2>\n");
   std::fprintf(fp, "# bucket <side> <b> <min ratings> <max ratings> variant <1|2> persistent <wavefronts>"
-                  " solver <-1 handle's | 0 Cholesky | 1 CG>\n");
+                  " solver <-1 handle's | 0 Cholesky | 1 CG> occupancy <wavefronts per CU, 0 any>\n");
   for (int side = 0; side < 2; ++side) {
     const Plan& p = side == 0 ? h->plan_x : h->plan_t;
     std::fprintf(fp, "side %d split %d finish %d fallback %d\n", side, p.split > 0 ? p.split : h->opt.split,
                  h->finish_variant[side], h->fallback_variant[side]);
     for (int b = 0; b < kBuckets; ++b)
-      std::fprintf(fp, "bucket %d %d %lld %lld variant %d persistent %d solver %d\n", side, b,
+      std::fprintf(fp, "bucket %d %d %lld %lld variant %d persistent %d solver %d occupancy %d\n", side, b,
                    (long long)kBucketEdge[b], (long long)(b + 1 < kBuckets ? kBucketEdge[b + 1] - 1 : -1),
-                   h->tiles[side][b].variant, h->tiles[side][b].persistent, h->tiles[side][b].solver);
+                   h->tiles[side][b].variant, h->tiles[side][b].persistent, h->tiles[side][b].solver,
+                   h->tiles[side][b].occupancy);
   }
   const bool ok = std::fclose(fp) == 0;
   if (!ok) cumf::set_last_error(std::string("cannot write ") + path);
@@ -1252,20 +1263,22 @@ int cumf_load_tile_table(cumf_handle* h, const char* path) {
   while (rc == CUMF_OK && std::fgets(line, sizeof line, fp)) {
     ++lineno;
     if (line[0] == '#' || line[0] == '\n') continue;
-    int side, sp, fv, bv, b, v, pers, sol = -1, nf = 0;
+    int side, sp, fv, bv, b, v, pers, sol = -1, occ = 0, nf = 0;
     long long lo, hi;
     if (std::sscanf(line, "side %d split %d finish %d fallback %d", &side, &sp, &fv, &bv) == 4) {
       if (side < 0 || side > 1 || sp < 1 || (fv != 1 && fv != 2) || (bv != 1 && bv != 2)) rc = CUMF_ERR_INVALID;
       else { split[side] = sp; fin[side] = fv; fb[side] = bv; }
-    } else if ((nf = std::sscanf(line, "bucket %d %d %lld %lld variant %d persistent %d solver %d", &side, &b, &lo,
-                                 &hi, &v, &pers, &sol)) >= 6) {
+    } else if ((nf = std::sscanf(line, "bucket %d %d %lld %lld variant %d persistent %d solver %d occupancy %d",
+                                 &side, &b, &lo, &hi, &v, &pers, &sol, &occ)) >= 6) {
       // A bucket may run the CG solver only on a handle built for CG (its
       // fallback list and kernel); Cholesky is always available.
       if (nf == 6) sol = -1;
+      if (nf <= 7) occ = 0;
       if (side < 0 || side > 1 || b < 0 || b >= kBuckets || (v != 1 && v != 2) || pers < 0 || pers > (1 << 20) ||
-          sol < -1 || sol > 1 || (sol == CUMF_SOLVER_CG && h->opt.solver != CUMF_SOLVER_CG))
+          sol < -1 || sol > 1 || (sol == CUMF_SOLVER_CG && h->opt.solver != CUMF_SOLVER_CG) || occ < 0 ||
+          occ > cumf::kMaxOccupancy || (occ > 0 && occ < cumf::kMinOccupancy))
         rc = CUMF_ERR_INVALID;
-      else tiles[side][b] = {v, pers, sol};
+      else tiles[side][b] = {v, pers, sol, occ};
     } else {
       rc = CUMF_ERR_INVALID;
     }

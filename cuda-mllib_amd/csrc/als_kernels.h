@@ -68,7 +68,17 @@ struct SweepArgs {
   // *task_counter (zero before the launch) until ntasks are taken.
   int persistent;
   int* task_counter;
+  // occupancy > 0 (one-task-per-group launches): at most that many
+  // wavefronts per CU, enforced through the LDS each group requests.
+  int occupancy;
 };
+
+// LDS of one CU, and the range of occupancy caps a launch accepts.
+constexpr int kLdsBytesPerCU = 160 * 1024;
+constexpr int kMinOccupancy = 3, kMaxOccupancy = 16;
+// LDS a group requests so that at most n groups fit one CU: more than a
+// (n+1)-th of the CU's LDS, in 512-byte steps.
+inline size_t occupancy_lds_bytes(int n) { return (size_t)(kLdsBytesPerCU / (n + 1) / 512 + 1) * 512; }
 
 // The reference-reproducing path (compat.hip): one work-group per row.
 struct CompatArgs {

@@ -184,7 +184,11 @@ template <int NB>
 static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which) {
   constexpr bool CG = CUMF_SWEEP_SOLVER == 1;
   const bool bf = a.precision != 0;
-  const size_t shmem = sizeof(float) * (size_t)cmax(lds_floats_nb(NB, bf), CG ? cg_lds_floats(NB, a.f) : 0);
+  size_t shmem = sizeof(float) * (size_t)cmax(lds_floats_nb(NB, bf), CG ? cg_lds_floats(NB, a.f) : 0);
+  if (which == 0 && a.persistent == 0 && a.occupancy > 0) {
+    if (a.occupancy < kMinOccupancy || a.occupancy > kMaxOccupancy) return hipErrorInvalidValue;
+    if (occupancy_lds_bytes(a.occupancy) > shmem) shmem = occupancy_lds_bytes(a.occupancy);
+  }
   if (which == 0) {
     if (a.ntasks > 0) {
       if (a.persistent > 0) {

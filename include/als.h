@@ -246,8 +246,10 @@ int cumf_csr_to_csc_device(hipStream_t stream, int64_t m, int64_t n, int64_t nnz
  * table gives, per side (0 = X, 1 = Theta): the split size, the register
  * budget (1 or 2 wavefronts per SIMD) of the split-row finish and of the CG
  * fallback kernels, and per bucket the register budget, whether the bucket
- * runs as a persistent grid of N wavefronts taking tasks from a counter, and
- * its solver (the handle's, Cholesky, or -- on a CG handle -- CG).  Without
+ * runs as a persistent grid of N wavefronts taking tasks from a counter, its
+ * solver (the handle's, Cholesky, or -- on a CG handle -- CG), and a cap on
+ * the wavefronts per CU of a one-task-per-group launch (3..16; 0 = none),
+ * enforced through the LDS each group requests.  Without
  * a table every half-sweep is one launch of all tasks,
  * with two wavefronts per SIMD up to f = 124 and one above; a side whose
  * non-empty buckets share one budget and no persistent grid is also one
@@ -256,8 +258,9 @@ int cumf_csr_to_csc_device(hipStream_t stream, int64_t m, int64_t n, int64_t nnz
 int cumf_load_tile_table(cumf_handle* h, const char* path);
 int cumf_save_tile_table(cumf_handle* h, const char* path);
 /* Autotuner: on the loaded ratings, times each split size per side, then
- * every launch shape and solver of every non-empty bucket (min over reps
- * runs), and keeps the fastest; the per-bucket shapes are kept only if a
+ * every launch shape (register budget, persistent grid or occupancy cap)
+ * and solver of every non-empty bucket (min over reps runs), and keeps the
+ * fastest; the per-bucket shapes are kept only if a
  * whole half-sweep with them beats one launch of every task with the
  * default shape (with the handle's solver, or with one solver for the whole
  * side).  The factors are restored afterwards, also on failure.  Writes the

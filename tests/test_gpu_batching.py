@@ -36,12 +36,15 @@ def _rel(a, b):
     return float(np.max(np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-30)))
 
 
-def _table(tmp_path, split, variant_of, persistent_of):
+def _table(tmp_path, split, variant_of, persistent_of, occupancy_of=None):
     lines = ["# test table"]
     for side in (0, 1):
         lines.append(f"side {side} split {split} finish {1 + side} fallback {2 - side}")
         for b in range(8):
-            lines.append(f"bucket {side} {b} 0 0 variant {variant_of(side, b)} persistent {persistent_of(side, b)}")
+            line = f"bucket {side} {b} 0 0 variant {variant_of(side, b)} persistent {persistent_of(side, b)}"
+            if occupancy_of is not None:
+                line += f" solver -1 occupancy {occupancy_of(side, b)}"
+            lines.append(line)
     p = tmp_path / "tiles.txt"
     p.write_text("\n".join(lines) + "\n")
     return p
@@ -60,6 +63,21 @@ def test_bucket_shapes_match_single_launch(tmp_path, solver, precision):
     assert _rel(Xb, Xr) <= 1e-6 and _rel(Tb, Tr) <= 1e-6, (_rel(Xb, Xr), _rel(Tb, Tr))
     # one launch per non-empty bucket and half-sweep
     assert st["task_launches"] > 4
+
+
+def test_occupancy_caps_match_single_launch(tmp_path):
+    """Buckets launched one task per group with at most 3, 6 or 7 wavefronts
+    per CU (the LDS each group requests) give the factors of one uncapped
+    launch."""
+    ds = _data.netflix_like_small()
+    f, lam = 100, 0.058
+    T0 = cumf.init_factors(ds.n, f, 45)
+    Xr, Tr, _ = _run(ds, f, lam, T0, split=512, solver=G.CUMF_SOLVER_CG, precision=G.CUMF_PREC_BF16_GRAM)
+    tab = _table(tmp_path, 512, lambda s, b: 1 if b % 4 == 3 else 2, lambda s, b: 0,
+                 lambda s, b: 3 if b % 4 == 3 else (0, 6, 7)[b % 3])
+    Xc, Tc, st = _run(ds, f, lam, T0, table=tab, solver=G.CUMF_SOLVER_CG, precision=G.CUMF_PREC_BF16_GRAM)
+    assert st["failed_rows"] == 0
+    assert _rel(Xc, Xr) <= 1e-6 and _rel(Tc, Tr) <= 1e-6, (_rel(Xc, Xr), _rel(Tc, Tr))
 
 
 def test_all_persistent_buckets_and_graph_replay(tmp_path):
@@ -130,6 +148,10 @@ def test_bad_tile_table_rejected(tmp_path):
         p.write_text("bucket 0 9 0 0 variant 2 persistent 0\n")
         with pytest.raises(G.CumfError):
             h.load_tile_table(str(p))
+        for occ in (2, 17, -1):  # occupancy caps outside 3..16 (0 = none)
+            p.write_text(f"bucket 0 1 0 0 variant 2 persistent 0 solver -1 occupancy {occ}\n")
+            with pytest.raises(G.CumfError):
+                h.load_tile_table(str(p))
 
 
 def test_uniform_table_is_one_launch(tmp_path):
