@@ -280,16 +280,27 @@ hipError_t finish_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
 hipError_t fallback_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
   return variant == 1 ? cumf::w1::chol::launch_fallback_kernel(a, s) : cumf::w2::chol::launch_fallback_kernel(a, s);
 }
+hipError_t dual_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
+  return variant == 1 ? cumf::w1::chol::launch_dual_kernel(a, s) : cumf::w2::chol::launch_dual_kernel(a, s);
+}
 
-// Launch of one degree bucket's tasks with its shape.
+// Degree buckets whose rows may take the dual form (every task at most 63
+// ratings): 0 and 1.
+constexpr int kDualBuckets = 2;
+static_assert(kBucketEdge[kDualBuckets] == 64, "dual-form buckets hold at most 63 ratings");
+
+// Launch of the tasks of degree buckets b_lo .. b (a contiguous range of the
+// task list: tasks are ordered by decreasing rating count) with one shape.
 hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs& a, int b,
-                         const BucketCfg& cfg) {
+                         const BucketCfg& cfg, int b_lo = -1) {
+  if (b_lo < 0) b_lo = b;
   cumf::SweepArgs x = a;
   x.tasks = a.tasks + p.bucket_t0[b];
-  x.ntasks = p.bucket_t1[b] - p.bucket_t0[b];
+  x.ntasks = p.bucket_t1[b_lo] - p.bucket_t0[b];
   x.persistent = cfg.persistent;
   x.occupancy = cfg.persistent ? 0 : cfg.occupancy;
   x.task_counter = h->counters.as<int>() + side * kBuckets + b;
+  if (cfg.solver == CUMF_SOLVER_DUAL) return dual_launch(x, cfg.variant, h->stream);
   if (cfg.solver >= 0) x.solver = cfg.solver;
   return task_launch(x, cfg.variant, h->stream);
 }
@@ -314,6 +325,8 @@ int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs&
     }
     uniform = uniform && c.persistent == 0 && c.variant == v0 && sv == s0 && c.occupancy == o0;
   }
+  if (uniform && v0 != 0 && s0 == CUMF_SOLVER_DUAL)
+    return timed(h, Timer::TASK, [&] { return dual_launch(a, v0, h->stream); });
   if (uniform && v0 != 0) {
     cumf::SweepArgs x = a;
     x.solver = s0;
@@ -324,9 +337,20 @@ int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs&
   for (int b = 0; b < kBuckets; ++b) persistent = persistent || h->tiles[side][b].persistent > 0;
   if (persistent)
     CUMF_HIP_TRY(cumf::launch_fallback_reset(h->counters.as<int>() + side * kBuckets, kBuckets, h->stream));
+  // Heaviest first; neighbouring non-empty buckets with one shape (and no
+  // persistent grid) share a launch.
+  auto same = [](const BucketCfg& x, const BucketCfg& y) {
+    return x.persistent == 0 && y.persistent == 0 && x.variant == y.variant && x.solver == y.solver &&
+           x.occupancy == y.occupancy;
+  };
   for (int b = kBuckets - 1; b >= 0; --b) {
     if (p.bucket_t1[b] <= p.bucket_t0[b]) continue;
-    CUMF_TRY(timed(h, Timer::TASK, [&] { return bucket_launch(h, side, p, a, b, h->tiles[side][b]); }));
+    int lo = b;
+    while (lo > 0 && (p.bucket_t1[lo - 1] <= p.bucket_t0[lo - 1] || same(h->tiles[side][lo - 1], h->tiles[side][b])))
+      --lo;
+    while (p.bucket_t1[lo] <= p.bucket_t0[lo]) ++lo;  // the range ends at a non-empty bucket
+    CUMF_TRY(timed(h, Timer::TASK, [&] { return bucket_launch(h, side, p, a, b, h->tiles[side][b], lo); }));
+    b = lo;
   }
   return CUMF_OK;
 }
@@ -1116,13 +1140,16 @@ int autotune_side(cumf_handle* h, int side, int reps, const TuneStart& start) {
     if (b.p->bucket_t1[k] <= b.p->bucket_t0[k]) continue;
     double bk = 1e300;
     BucketCfg bc;
-    for (int sol : solvers)
+    std::vector<int> sols = solvers;
+    if (h->opt.precision != CUMF_PREC_FP32 && k < kDualBuckets) sols.push_back(CUMF_SOLVER_DUAL);
+    for (int sol : sols)
       for (int variant : {2, 1})
         for (int pers : {0, h->num_cus * (variant == 1 ? 4 : 8), -7, -6, -3}) {
           // pers < 0: a one-task-per-group launch capped at -pers wavefronts
           // per CU (7 or 6 of the eight a two-wavefront budget allows, 3 of
           // the four of a one-wavefront budget)
           if (pers < 0 && (variant == 1) != (pers == -3)) continue;
+          if (sol == CUMF_SOLVER_DUAL && pers != 0) continue;  // one task per group, uncapped
           BucketCfg cfg{variant, std::max(pers, 0), sol, std::max(-pers, 0)};
           double ms = 0;
           CUMF_TRY(time_min_ms(h, reps, restore, [&] {
@@ -1274,9 +1301,12 @@ int cumf_load_tile_table(cumf_handle* h, const char* path) {
       // fallback list and kernel); Cholesky is always available.
       if (nf == 6) sol = -1;
       if (nf <= 7) occ = 0;
+      // The dual form needs BF16 factors and rows of at most 63 ratings
+      // (buckets 0 and 1), one task per group.
       if (side < 0 || side > 1 || b < 0 || b >= kBuckets || (v != 1 && v != 2) || pers < 0 || pers > (1 << 20) ||
-          sol < -1 || sol > 1 || (sol == CUMF_SOLVER_CG && h->opt.solver != CUMF_SOLVER_CG) || occ < 0 ||
-          occ > cumf::kMaxOccupancy || (occ > 0 && occ < cumf::kMinOccupancy))
+          sol < -1 || sol > 2 || (sol == CUMF_SOLVER_CG && h->opt.solver != CUMF_SOLVER_CG) ||
+          (sol == CUMF_SOLVER_DUAL && (h->opt.precision == CUMF_PREC_FP32 || b >= kDualBuckets || pers > 0)) ||
+          occ < 0 || occ > cumf::kMaxOccupancy || (occ > 0 && occ < cumf::kMinOccupancy))
         rc = CUMF_ERR_INVALID;
       else tiles[side][b] = {v, pers, sol, occ};
     } else {

@@ -1454,5 +1454,143 @@ __device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepA
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dual (kernel) form for rows with few ratings, BF16 factors.  With Y the
+// n x f matrix of the row's gathered factor rows, r its ratings and
+// mu = lambda n, the push-through identity
+//     (Y^T Y + mu I)^{-1} Y^T r = Y^T (Y Y^T + mu I)^{-1} r
+// gives the same x from an n x n system: z solves (Y Y^T + mu I) z = r and
+// x = Y^T z.  For n <= kDualMaxRatings (63) that system has at most four
+// 16-column blocks against NB for the f x f one, so the Cholesky costs a
+// fraction of the primal one (the primal Gram's MFMA work is about the
+// same as the dual's).
+//   * Y is gathered as in gram_range_bf16: chunks of 32 rows, NB tiles of
+//     32 rows x 16 bf16, both chunks at once (rows past n are zeros).
+//   * K = Y Y^T on v_mfma_f32_16x16x32_bf16: for k-step s and row block I,
+//     lane (kg, i) supplies row 16I + i, columns 32s + 8kg .. +7 (one
+//     16-byte LDS read; columns past the staged tiles read a zero block).
+//     The same read is the A operand of block I and the B operand of block
+//     J = I.
+//   * r goes into column 16 NBD - 1 (past every live pivot), mu onto the
+//     first n diagonal entries, and chol_solve factors the NBD blocks and
+//     writes z to LDS.
+//   * x = Y^T z in FP32 on the VALU: lane l forms columns 2l, 2l+1 from the
+//     factor rows in memory (L2-resident on the X side), rating by rating.
+// ---------------------------------------------------------------------------
+constexpr int kDualMaxRatings = 2 * KBB - 1;
+
+template <int NB>
+__host__ __device__ constexpr int dual_lds_floats() {
+  // the two staged chunks, then z (64), the ratings (64) and a zero block (4)
+  return cmax(2 * NB * 256, chol_lds_floats(4)) + 64 + 64 + 4;
+}
+
+template <int NB, int NBD>
+__device__ __forceinline__ bool dual_blocks(const SweepArgs& a, int n, float* lds, int lane) {
+  constexpr int TB = KBB * 16, CB = NB * TB;  // bf16 per staged tile / chunk
+  constexpr int S = (NB + 1) / 2;             // 32-column k-steps over the staged tiles
+  const __bf16* yb = reinterpret_cast<const __bf16*>(lds);
+  float* zs = lds + cmax(2 * NB * 256, chol_lds_floats(4));
+  const float* rs = zs + 64;
+  const __bf16* zero = reinterpret_cast<const __bf16*>(zs + 128);
+  const int kg = lane >> 4, i = lane & 15;
+  f4 acc[num_tiles(NBD)];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int t = 2 * s + (kg >> 1);  // tile of this lane's 8 columns
+    bf8 op[NBD];
+#pragma unroll
+    for (int I = 0; I < NBD; ++I) {
+      const int r = 16 * I + i;
+      const __bf16* p = t < NB ? yb + (r >> 5) * CB + t * TB + (r & 31) * 16 + 8 * (kg & 1) : zero;
+      op[I] = *reinterpret_cast<const bf8*>(p);
+    }
+#pragma unroll
+    for (int I = 0; I < NBD; ++I)
+#pragma unroll
+      for (int J = I; J < NBD; ++J)
+        acc[tile_index(I, J, NBD)] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            op[I], op[J], s == 0 ? f4{0.f, 0.f, 0.f, 0.f} : acc[tile_index(I, J, NBD)], 0, 0, 0);
+  }
+  // r into column 16 NBD - 1 of the last block column (K is zero there:
+  // rows of Y past n are zero and 16 NBD - 1 >= n), mu on the diagonal.
+  const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+  for (int I = 0; I < NBD; ++I)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c == 15) acc[tile_index(I, NBD - 1, NBD)][q] = rs[16 * I + 4 * g + q];
+  add_ridge<NBD>(acc, n, a.lambda * (float)n, lane);
+  lds_order();  // the staged rows are dead: chol_solve reuses their space
+  return chol_solve<NBD>(acc, n, 16 * NBD - 1, lds, zs, 16 * NBD, lane);
+}
+
+template <int NB>
+__device__ __forceinline__ void dual_solve_row(const SweepArgs& a, const Task& t, float* lds, int lane) {
+  constexpr int TB = KBB * 16, CB = NB * TB;
+  float* xrow = a.x + (int64_t)t.row * a.ldf;
+  __bf16* xb = reinterpret_cast<__bf16*>(a.x_b) + (int64_t)t.row * a.ldb;
+  const int n = t.count;
+  if (n <= 0 || n > kDualMaxRatings) {  // (the host launches only rows of at most 63 ratings)
+    for (int i = lane; i < a.ldf; i += 64) xrow[i] = 0.0f;
+    for (int i = lane; i < a.ldb; i += 64) xb[i] = (__bf16)0.0f;
+    if (n > kDualMaxRatings && lane == 0) atomicAdd(a.fail_counter, 1);
+    return;
+  }
+  const __bf16* theta_b = reinterpret_cast<const __bf16*>(a.theta_b);
+  __bf16* yb = reinterpret_cast<__bf16*>(lds);
+  float* zs = lds + cmax(2 * NB * 256, chol_lds_floats(4));
+  float* rs = zs + 64;
+  // Gather both chunks: lane l moves row l/2 of the chunk, 16 bytes of each
+  // tile (rows past n and slots past ldb come from kZeroRow).
+  {
+    const int sr = lane >> 1, sh = lane & 1;
+    const bool last_in = 16 * (NB - 1) + 8 * sh < (int)a.ldb;
+    const __bf16* zb = reinterpret_cast<const __bf16*>(kZeroRow);
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      const int k = ch * KBB + sr;
+      const __bf16* rp = k < n ? theta_b + (int64_t)a.col[t.start + k] * a.ldb + 8 * sh : zb;
+      __bf16* dst = yb + ch * CB;
+#pragma unroll
+      for (int I = 0; I < NB - 1; ++I)
+        __builtin_amdgcn_global_load_lds(rp + 16 * I, (lds_void_t*)(dst + I * TB), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(last_in && k < n ? rp + 16 * (NB - 1) : zb,
+                                       (lds_void_t*)(dst + (NB - 1) * TB), 16, 0, 0);
+    }
+    rs[lane] = lane < n ? a.val[t.start + lane] : 0.0f;
+    if (lane < 4) zs[128 + lane] = 0.0f;  // the zero block
+    wait_vm_all();
+    lds_order();
+  }
+  bool ok;
+  const int nbd = (n + 16) >> 4;  // blocks of the n x n system plus r's column
+  if (nbd == 1) ok = dual_blocks<NB, 1>(a, n, lds, lane);
+  else if (nbd == 2) ok = dual_blocks<NB, 2>(a, n, lds, lane);
+  else if (nbd == 3) ok = dual_blocks<NB, 3>(a, n, lds, lane);
+  else ok = dual_blocks<NB, 4>(a, n, lds, lane);
+  lds_order();
+  // x = Y^T z; chol_solve left z = 0 if it failed.  Lane l forms columns
+  // c0 = 2l + 128 p, c0 + 1.
+  for (int c0 = 2 * lane; c0 < (int)a.ldb; c0 += 128) {
+    const __bf16* base = theta_b + c0;
+    float x0 = 0.0f, x1 = 0.0f;
+#pragma unroll 4
+    for (int k = 0; k < n; ++k) {
+      const int ci = a.col[t.start + k];
+      const uint32_t w = *reinterpret_cast<const uint32_t*>(base + (int64_t)ci * a.ldb);
+      const float zk = zs[k];
+      x0 = fmaf(zk, __uint_as_float(w << 16), x0);
+      x1 = fmaf(zk, __uint_as_float(w & 0xffff0000u), x1);
+    }
+    x0 = c0 < a.f ? x0 : 0.0f;
+    x1 = c0 + 1 < a.f ? x1 : 0.0f;
+    if (c0 < (int)a.ldf) xrow[c0] = x0;
+    if (c0 + 1 < (int)a.ldf) xrow[c0 + 1] = x1;
+    xb[c0] = (__bf16)x0;
+    xb[c0 + 1] = (__bf16)x1;
+  }
+  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
+}
 
 }  // namespace cumf

@@ -106,12 +106,14 @@ hipError_t launch_fallback_reset(int* count, int n, hipStream_t s);
 // w2 = two wavefronts per SIMD, w1 = one.
 // Launchers of the sweep kernels, per register budget (w1, w2) and solver
 // family (chol: Cholesky task/finish kernels and the CG path's fallback;
-// cg: CG task/finish kernels).
+// cg: CG task/finish kernels; chol also holds the dual-form task kernel
+// for rows of at most 63 ratings, BF16 factors only).
 #define CUMF_SWEEP_LAUNCHERS                                                \
   namespace chol {                                                          \
   hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s);         \
   hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s);       \
   hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s);     \
+  hipError_t launch_dual_kernel(const SweepArgs& a, hipStream_t s);         \
   }                                                                         \
   namespace cg {                                                            \
   hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s);         \

@@ -177,6 +177,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
   }
 }
 
+#if CUMF_SWEEP_SOLVER == 0
+// Dual form of the tasks of one launch, each of at most kDualMaxRatings
+// ratings (BF16 factors).  The last chunk of a split row can be that short:
+// such a task still forms its partial Gram.
+template <int NB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_dual_kernel(SweepArgs a) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const Task t = a.tasks[blockIdx.x];
+  if (t.slot >= 0) {
+    f4 acc[num_tiles(NB)];
+    gram_any<NB, true>(acc, a, (int64_t)t.start, t.count, lds, lane);
+    float* dst = a.partial + (int64_t)t.slot * num_tiles(NB) * 256;
+#pragma unroll
+    for (int i = 0; i < num_tiles(NB); ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dst[(i * 4 + r) * 64 + lane] = acc[i][r];
+    return;
+  }
+  dual_solve_row<NB>(a, t, lds, lane);
+}
+#endif
 
 constexpr int kFallbackGrid = 2048;  // work-groups walking the fallback list
 
@@ -205,10 +227,21 @@ static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which) {
     if (a.nheavy > 0)
       hipLaunchKernelGGL((als_finish_kernel<NB, CG>), dim3((unsigned)a.nheavy), dim3(64), shmem, s, a);
   } else if constexpr (!CG) {
-    // which == 3: the CG path's Cholesky fallback (lives with the Cholesky kernels)
-    const size_t shc = sizeof(float) * (size_t)lds_floats_nb(NB, bf);
-    if (bf) hipLaunchKernelGGL((als_fallback_kernel<NB, true>), dim3(kFallbackGrid), dim3(64), shc, s, a);
-    else hipLaunchKernelGGL((als_fallback_kernel<NB, false>), dim3(kFallbackGrid), dim3(64), shc, s, a);
+    if (which == 3) {
+      // the CG path's Cholesky fallback (lives with the Cholesky kernels)
+      const size_t shc = sizeof(float) * (size_t)lds_floats_nb(NB, bf);
+      if (bf) hipLaunchKernelGGL((als_fallback_kernel<NB, true>), dim3(kFallbackGrid), dim3(64), shc, s, a);
+      else hipLaunchKernelGGL((als_fallback_kernel<NB, false>), dim3(kFallbackGrid), dim3(64), shc, s, a);
+    } else {
+#if CUMF_SWEEP_SOLVER == 0
+      // which == 4: the dual form (BF16 factors only)
+      if (!bf) return hipErrorInvalidValue;
+      if (a.ntasks > 0) {
+        const size_t shd = sizeof(float) * (size_t)cmax(dual_lds_floats<NB>(), lds_floats_nb(NB, true));
+        hipLaunchKernelGGL((als_dual_kernel<NB>), dim3((unsigned)a.ntasks), dim3(64), shd, s, a);
+      }
+#endif
+    }
   } else {
     return hipErrorInvalidValue;
   }
@@ -234,6 +267,7 @@ hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s) { return dispat
 hipError_t launch_finish_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 1); }
 #if CUMF_SWEEP_SOLVER == 0
 hipError_t launch_fallback_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 3); }
+hipError_t launch_dual_kernel(const SweepArgs& a, hipStream_t s) { return dispatch(a, s, 4); }
 #endif
 
 }  // namespace CUMF_SNS

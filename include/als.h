@@ -84,6 +84,12 @@ void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, f
                                    current factor; rows whose relative residual
                                    stays above cg_tol fall back to the Cholesky
                                    solve */
+#define CUMF_SOLVER_DUAL 2      /* tile tables only, degree buckets 0 and 1 (at
+                                   most 63 ratings) of a CUMF_PREC_BF16_GRAM
+                                   handle: the same x from the n x n system
+                                   (Y Y^T + lambda n I) z = r, x = Y^T z, with
+                                   Y the row's n gathered factor rows (FP32
+                                   Cholesky of the smaller system) */
 
 #define CUMF_WARM_PREV 0        /* CG starts from the row's current factor */
 #define CUMF_WARM_ZERO 1        /* CG starts from zero */

@@ -80,6 +80,55 @@ def test_occupancy_caps_match_single_launch(tmp_path):
     assert _rel(Xc, Xr) <= 1e-6 and _rel(Tc, Tr) <= 1e-6, (_rel(Xc, Xr), _rel(Tc, Tr))
 
 
+def _dual_table(tmp_path, split, dual_variant=2):
+    lines = []
+    for side in (0, 1):
+        lines.append(f"side {side} split {split} finish 2 fallback 2")
+        for b in range(8):
+            sol = 2 if b < 2 else -1
+            lines.append(f"bucket {side} {b} 0 0 variant {dual_variant if b < 2 else 2} persistent 0 solver {sol}")
+    p = tmp_path / "dual.txt"
+    p.write_text("\n".join(lines) + "\n")
+    return p
+
+
+@pytest.mark.parametrize("f", [32, 100])
+def test_dual_form_matches_primal_and_oracle(tmp_path, f):
+    """Rows of at most 63 ratings solved in the dual form, (Y Y^T + mu I) z = r
+    and x = Y^T z, give the primal solution: within 1e-4 of the BF16 primal
+    path, residual <= 1e-5 against the FP64 system on the BF16 factors.
+    With split 48 every chunk of a longer row also lands in buckets 0-1: the
+    dual kernel forms those partial Grams, and those rows match the primal
+    path bit for bit."""
+    ds = _data.netflix_like_small()
+    lam = 0.058
+    T0 = cumf.init_factors(ds.n, f, 46)
+    rp = ds.csr_row_ptr
+    deg = np.diff(rp)
+    for split in (16384, 48):
+        opt = dict(precision=G.CUMF_PREC_BF16_GRAM, solver=G.CUMF_SOLVER_CHOLESKY, split=split)
+        Xp, Tp, _ = _run(ds, f, lam, T0, iters=1, **opt)
+        Xd, Td, st = _run(ds, f, lam, T0, iters=1, table=_dual_table(tmp_path, split, 1 if f == 32 else 2), **opt)
+        assert st["failed_rows"] == 0
+        light = deg <= 63
+        assert light.sum() > 100
+        assert _rel(Xd[light], Xp[light]) <= 1e-4, _rel(Xd[light], Xp[light])
+        if split == 48:
+            heavy = ~light
+            np.testing.assert_array_equal(Xd[heavy], Xp[heavy])
+    # one X half-sweep from T0 checked against the FP64 oracle on light rows
+    with G.ALS(f, lam, precision=G.CUMF_PREC_BF16_GRAM, solver=G.CUMF_SOLVER_CHOLESKY) as h:
+        h.set_dataset(ds)
+        h.load_tile_table(str(_dual_table(tmp_path, 16384)))
+        h.set_factors(theta=T0)
+        h.update_x()
+        X1, _ = h.get_factors()
+    from tests._checks import bf16_round, row_errors
+    rows = np.flatnonzero((deg > 0) & (deg <= 63))[::7][:200]
+    res, err = row_errors(rp, ds.csr_col, ds.csr_val, bf16_round(T0), X1, f, lam, rows)
+    assert res.max() <= 1e-5 and err.max() <= 1e-3, (res.max(), err.max())
+
+
 def test_all_persistent_buckets_and_graph_replay(tmp_path):
     ds = _data.netflix_like_small()
     f, lam = 32, 0.058
@@ -152,6 +201,19 @@ def test_bad_tile_table_rejected(tmp_path):
             p.write_text(f"bucket 0 1 0 0 variant 2 persistent 0 solver -1 occupancy {occ}\n")
             with pytest.raises(G.CumfError):
                 h.load_tile_table(str(p))
+        # the dual form: FP32 handle, a bucket that may hold 64+ ratings, a persistent grid
+        for line in ("bucket 0 1 0 0 variant 2 persistent 0 solver 2", "bucket 0 2 0 0 variant 2 persistent 0 solver 2"):
+            p.write_text(line + "\n")
+            with pytest.raises(G.CumfError):
+                h.load_tile_table(str(p))
+    with G.ALS(16, 0.05, precision=G.CUMF_PREC_BF16_GRAM) as h:
+        for line in ("bucket 0 2 0 0 variant 2 persistent 0 solver 2", "bucket 1 0 0 0 variant 2 persistent 64 solver 2",
+                     "bucket 0 0 0 0 variant 2 persistent 0 solver 3"):
+            p.write_text(line + "\n")
+            with pytest.raises(G.CumfError):
+                h.load_tile_table(str(p))
+        p.write_text("bucket 0 1 0 0 variant 2 persistent 0 solver 2\n")
+        h.load_tile_table(str(p))
 
 
 def test_uniform_table_is_one_launch(tmp_path):
