@@ -49,6 +49,7 @@ int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }
 // [kBucketEdge[b], kBucketEdge[b + 1]) form bucket b.
 constexpr int kBuckets = 8;
 constexpr int64_t kBucketEdge[kBuckets + 1] = {0, 16, 64, 256, 1024, 4096, 16384, 65536, INT64_MAX};
+constexpr int kAuxStreams = 3;
 
 // Launch shape of one bucket: register budget (wavefronts per SIMD the
 // kernel was compiled for, 1 or 2), persistent grid (0: one work-group per
@@ -142,7 +143,7 @@ int check_f(int f) {
 // Kernel-time bookkeeping with HIP events.
 struct Timer {
   enum Kind { TASK = 0, FINISH, RMSE, SWEEP, ITER, WARM, FALLBACK, NKINDS };
-  struct Pending { hipEvent_t a, b; int kind; };
+  struct Pending { hipEvent_t a, b; int kind; int launches; };
   std::vector<hipEvent_t> pool;
   std::vector<Pending> pending;
   double ms[NKINDS] = {};
@@ -162,7 +163,7 @@ struct Timer {
   void resolve() {
     for (auto& p : pending) {
       float t = 0.f;
-      if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) { ms[p.kind] += t; count[p.kind]++; }
+      if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) { ms[p.kind] += t; count[p.kind] += p.launches; }
       else (void)hipGetLastError();
       pool.push_back(p.a);
       pool.push_back(p.b);
@@ -180,6 +181,11 @@ struct cumf_handle {
   float lambda = 0.f;
   cumf_options opt{};
   hipStream_t stream = nullptr;
+  // Streams (default priority) on which the launches of a half-sweep's
+  // degree buckets run beside the first one, forked from and joined back
+  // into `stream` with the events below (also under graph capture).
+  hipStream_t aux[kAuxStreams] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kAuxStreams] = {};
   int64_t m = 0, n = 0, nnz = 0, ntest = 0;
   DevBuf rp, ci, rv, cp, ri, cv;  // CSR and CSC
   DevBuf X, T;                    // factors, row stride ldf
@@ -221,6 +227,18 @@ struct cumf_handle {
       g = nullptr;
     }
   }
+  void destroy_streams() {
+    for (int i = 0; i < kAuxStreams; ++i) {
+      if (aux[i]) (void)hipStreamDestroy(aux[i]);
+      if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+      aux[i] = nullptr;
+      join_ev[i] = nullptr;
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    fork_ev = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
 };
 
 namespace {
@@ -240,7 +258,7 @@ struct DeviceGuard {
 };
 
 template <class F>
-int timed(cumf_handle* h, int kind, F&& launch) {
+int timed(cumf_handle* h, int kind, F&& launch, int launches = 1) {
   hipEvent_t a = nullptr, b = nullptr;
   if (h->opt.timing && !h->capturing) {
     a = h->timer.get();
@@ -256,7 +274,7 @@ int timed(cumf_handle* h, int kind, F&& launch) {
   }
   if (a && b) {
     (void)hipEventRecord(b, h->stream);
-    h->timer.pending.push_back({a, b, kind});
+    h->timer.pending.push_back({a, b, kind, launches});
   }
   return CUMF_OK;
 }
@@ -292,17 +310,18 @@ static_assert(kBucketEdge[kDualBuckets] == 64, "dual-form buckets hold at most 6
 // Launch of the tasks of degree buckets b_lo .. b (a contiguous range of the
 // task list: tasks are ordered by decreasing rating count) with one shape.
 hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs& a, int b,
-                         const BucketCfg& cfg, int b_lo = -1) {
+                         const BucketCfg& cfg, int b_lo = -1, hipStream_t stream = nullptr) {
   if (b_lo < 0) b_lo = b;
+  if (!stream) stream = h->stream;
   cumf::SweepArgs x = a;
   x.tasks = a.tasks + p.bucket_t0[b];
   x.ntasks = p.bucket_t1[b_lo] - p.bucket_t0[b];
   x.persistent = cfg.persistent;
   x.occupancy = cfg.persistent ? 0 : cfg.occupancy;
   x.task_counter = h->counters.as<int>() + side * kBuckets + b;
-  if (cfg.solver == CUMF_SOLVER_DUAL) return dual_launch(x, cfg.variant, h->stream);
+  if (cfg.solver == CUMF_SOLVER_DUAL) return dual_launch(x, cfg.variant, stream);
   if (cfg.solver >= 0) x.solver = cfg.solver;
-  return task_launch(x, cfg.variant, h->stream);
+  return task_launch(x, cfg.variant, stream);
 }
 
 // All tasks of a half-sweep: one launch, or with a tile table one launch
@@ -337,22 +356,46 @@ int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs&
   for (int b = 0; b < kBuckets; ++b) persistent = persistent || h->tiles[side][b].persistent > 0;
   if (persistent)
     CUMF_HIP_TRY(cumf::launch_fallback_reset(h->counters.as<int>() + side * kBuckets, kBuckets, h->stream));
-  // Heaviest first; neighbouring non-empty buckets with one shape (and no
-  // persistent grid) share a launch.
+  // Launch groups, heaviest first: neighbouring non-empty buckets with one
+  // shape (and no persistent grid) share a launch.
   auto same = [](const BucketCfg& x, const BucketCfg& y) {
     return x.persistent == 0 && y.persistent == 0 && x.variant == y.variant && x.solver == y.solver &&
            x.occupancy == y.occupancy;
   };
+  int grp_hi[kBuckets], grp_lo[kBuckets], ng = 0;
   for (int b = kBuckets - 1; b >= 0; --b) {
     if (p.bucket_t1[b] <= p.bucket_t0[b]) continue;
     int lo = b;
     while (lo > 0 && (p.bucket_t1[lo - 1] <= p.bucket_t0[lo - 1] || same(h->tiles[side][lo - 1], h->tiles[side][b])))
       --lo;
     while (p.bucket_t1[lo] <= p.bucket_t0[lo]) ++lo;  // the range ends at a non-empty bucket
-    CUMF_TRY(timed(h, Timer::TASK, [&] { return bucket_launch(h, side, p, a, b, h->tiles[side][b], lo); }));
+    grp_hi[ng] = b;
+    grp_lo[ng++] = lo;
     b = lo;
   }
-  return CUMF_OK;
+  // The groups run side by side: the first on the handle's stream, the
+  // others on the auxiliary streams forked from it and joined back (the
+  // heaviest rows no longer run alone at the end of a launch of their own).
+  return timed(h, Timer::TASK, [&]() -> hipError_t {
+    const int nstreams = std::min(ng, kAuxStreams + 1);
+    if (nstreams > 1) {
+      hipError_t e = hipEventRecord(h->fork_ev, h->stream);
+      for (int i = 0; e == hipSuccess && i + 1 < nstreams; ++i) e = hipStreamWaitEvent(h->aux[i], h->fork_ev, 0);
+      if (e != hipSuccess) return e;
+    }
+    for (int g = 0; g < ng; ++g) {
+      const int si = g % nstreams;
+      const hipStream_t st = si == 0 ? h->stream : h->aux[si - 1];
+      const hipError_t e = bucket_launch(h, side, p, a, grp_hi[g], h->tiles[side][grp_hi[g]], grp_lo[g], st);
+      if (e != hipSuccess) return e;
+    }
+    for (int i = 0; i + 1 < nstreams; ++i) {
+      hipError_t e = hipEventRecord(h->join_ev[i], h->aux[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->join_ev[i], 0);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }, ng);
 }
 
 bool uses_network(const cumf_handle* h) {
@@ -484,7 +527,7 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
   if (cg) CUMF_TRY(timed(h, Timer::FALLBACK, [&] { return fallback_launch(a, bv, h->stream); }));
   if (s0 && s1) {
     (void)hipEventRecord(s1, h->stream);
-    h->timer.pending.push_back({s0, s1, Timer::SWEEP});
+    h->timer.pending.push_back({s0, s1, Timer::SWEEP, 1});
   }
   h->gram_flops += p.nnz * (int64_t)h->f * (h->f + 1);
   return CUMF_OK;
@@ -604,7 +647,17 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
     cumf::set_last_error("hipStreamCreate failed");
     return CUMF_ERR_HIP;
   }
-  int rc = h->fail.alloc(2 * sizeof(int));  // failed rows, CG fallbacks
+  int rc = CUMF_OK;
+  for (int i = 0; i < kAuxStreams && rc == CUMF_OK; ++i)
+    if (hipStreamCreateWithFlags(&h->aux[i], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->join_ev[i], hipEventDisableTiming) != hipSuccess)
+      rc = CUMF_ERR_HIP;
+  if (rc == CUMF_OK && hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess) rc = CUMF_ERR_HIP;
+  if (rc != CUMF_OK) {
+    (void)hipGetLastError();
+    cumf::set_last_error("hipStreamCreate / hipEventCreate failed");
+  }
+  if (rc == CUMF_OK) rc = h->fail.alloc(2 * sizeof(int));  // failed rows, CG fallbacks
   if (rc == CUMF_OK) rc = h->counters.alloc(sizeof(int) * 2 * kBuckets);
   if (rc == CUMF_OK && hipDeviceGetAttribute(&h->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
     (void)hipGetLastError();
@@ -619,7 +672,7 @@ int cumf_create(int device, int f, float lambda, const cumf_options* opt, cumf_h
     rc = CUMF_ERR_HIP;
   }
   if (rc != CUMF_OK) {
-    (void)hipStreamDestroy(h->stream);
+    h->destroy_streams();
     delete h;
     return rc;
   }
@@ -634,8 +687,7 @@ int cumf_destroy(cumf_handle* h) {
     (void)hipStreamSynchronize(h->stream);
     h->timer.resolve();
     h->drop_graph();
-    (void)hipStreamDestroy(h->stream);
-    h->stream = nullptr;
+    h->destroy_streams();
     delete h;  // DevBuf destructors free on the handle's device
   }
   return CUMF_OK;
