@@ -1474,8 +1474,9 @@ __device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepA
 //   * r goes into column 16 NBD - 1 (past every live pivot), mu onto the
 //     first n diagonal entries, and chol_solve factors the NBD blocks and
 //     writes z to LDS.
-//   * x = Y^T z in FP32 on the VALU: lane l forms columns 2l, 2l+1 from the
-//     factor rows in memory (L2-resident on the X side), rating by rating.
+//   * x = Y^T z in FP32 on the VALU: the rows are gathered again (the
+//     Cholesky reused their LDS) and lane l forms columns 2l, 2l+1 from
+//     them, rating by rating.
 // ---------------------------------------------------------------------------
 constexpr int kDualMaxRatings = 2 * KBB - 1;
 
@@ -1542,27 +1543,34 @@ __device__ __forceinline__ void dual_solve_row(const SweepArgs& a, const Task& t
   float* zs = lds + cmax(2 * NB * 256, chol_lds_floats(4));
   float* rs = zs + 64;
   // Gather both chunks: lane l moves row l/2 of the chunk, 16 bytes of each
-  // tile (rows past n and slots past ldb come from kZeroRow).
-  {
-    const int sr = lane >> 1, sh = lane & 1;
-    const bool last_in = 16 * (NB - 1) + 8 * sh < (int)a.ldb;
-    const __bf16* zb = reinterpret_cast<const __bf16*>(kZeroRow);
+  // tile (rows past n and slots past ldb come from kZeroRow).  Done twice:
+  // for the Gram, and again after the Cholesky (which reuses the space) for
+  // x = Y^T z.
+  const int sr = lane >> 1, sh = lane & 1;
+  const bool last_in = 16 * (NB - 1) + 8 * sh < (int)a.ldb;
+  const __bf16* zb = reinterpret_cast<const __bf16*>(kZeroRow);
+  const __bf16* rp[2];
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    const int k = ch * KBB + sr;
+    rp[ch] = k < n ? theta_b + (int64_t)a.col[t.start + k] * a.ldb + 8 * sh : zb;
+  }
+  auto gather = [&] {
 #pragma unroll
     for (int ch = 0; ch < 2; ++ch) {
-      const int k = ch * KBB + sr;
-      const __bf16* rp = k < n ? theta_b + (int64_t)a.col[t.start + k] * a.ldb + 8 * sh : zb;
       __bf16* dst = yb + ch * CB;
 #pragma unroll
       for (int I = 0; I < NB - 1; ++I)
-        __builtin_amdgcn_global_load_lds(rp + 16 * I, (lds_void_t*)(dst + I * TB), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds(last_in && k < n ? rp + 16 * (NB - 1) : zb,
+        __builtin_amdgcn_global_load_lds(rp[ch] + 16 * I, (lds_void_t*)(dst + I * TB), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(last_in && rp[ch] != zb ? rp[ch] + 16 * (NB - 1) : zb,
                                        (lds_void_t*)(dst + (NB - 1) * TB), 16, 0, 0);
     }
-    rs[lane] = lane < n ? a.val[t.start + lane] : 0.0f;
-    if (lane < 4) zs[128 + lane] = 0.0f;  // the zero block
-    wait_vm_all();
-    lds_order();
-  }
+  };
+  gather();
+  rs[lane] = lane < n ? a.val[t.start + lane] : 0.0f;
+  if (lane < 4) zs[128 + lane] = 0.0f;  // the zero block
+  wait_vm_all();
+  lds_order();
   bool ok;
   const int nbd = (n + 16) >> 4;  // blocks of the n x n system plus r's column
   if (nbd == 1) ok = dual_blocks<NB, 1>(a, n, lds, lane);
@@ -1570,15 +1578,17 @@ __device__ __forceinline__ void dual_solve_row(const SweepArgs& a, const Task& t
   else if (nbd == 3) ok = dual_blocks<NB, 3>(a, n, lds, lane);
   else ok = dual_blocks<NB, 4>(a, n, lds, lane);
   lds_order();
-  // x = Y^T z; chol_solve left z = 0 if it failed.  Lane l forms columns
-  // c0 = 2l + 128 p, c0 + 1.
+  gather();
+  wait_vm_all();
+  lds_order();
+  // x = Y^T z from the staged rows (chol_solve left z = 0 if it failed).
+  // Lane l forms columns c0 = 2l + 128 p and c0 + 1 (tile c0 / 16).
   for (int c0 = 2 * lane; c0 < (int)a.ldb; c0 += 128) {
-    const __bf16* base = theta_b + c0;
+    const __bf16* col = yb + (c0 >> 4) * TB + (c0 & 15);
     float x0 = 0.0f, x1 = 0.0f;
-#pragma unroll 4
+#pragma unroll 8
     for (int k = 0; k < n; ++k) {
-      const int ci = a.col[t.start + k];
-      const uint32_t w = *reinterpret_cast<const uint32_t*>(base + (int64_t)ci * a.ldb);
+      const uint32_t w = *reinterpret_cast<const uint32_t*>(col + (k >> 5) * CB + (k & 31) * 16);
       const float zk = zs[k];
       x0 = fmaf(zk, __uint_as_float(w << 16), x0);
       x1 = fmaf(zk, __uint_as_float(w & 0xffff0000u), x1);
