@@ -48,7 +48,7 @@ int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }
 // Degree buckets of the batching form: tasks whose rating count falls in
 // [kBucketEdge[b], kBucketEdge[b + 1]) form bucket b.
 constexpr int kBuckets = 8;
-constexpr int64_t kBucketEdge[kBuckets + 1] = {0, 16, 64, 256, 1024, 4096, 16384, 65536, INT64_MAX};
+constexpr int64_t kBucketEdge[kBuckets + 1] = {0, 32, 64, 96, 256, 1024, 4096, 16384, INT64_MAX};
 constexpr int kAuxStreams = 3;
 
 // Launch shape of one bucket: register budget (wavefronts per SIMD the
@@ -302,10 +302,10 @@ hipError_t dual_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
   return variant == 1 ? cumf::w1::chol::launch_dual_kernel(a, s) : cumf::w2::chol::launch_dual_kernel(a, s);
 }
 
-// Degree buckets whose rows may take the dual form (every task at most 63
-// ratings): 0 and 1.
-constexpr int kDualBuckets = 2;
-static_assert(kBucketEdge[kDualBuckets] == 64, "dual-form buckets hold at most 63 ratings");
+// Degree buckets whose rows may take the dual form (every task at most 95
+// ratings): 0, 1 and 2.
+constexpr int kDualBuckets = 3;
+static_assert(kBucketEdge[kDualBuckets] == cumf::kDualMaxRatingsHost + 1, "dual-form buckets hold at most 95 ratings");
 
 // Launch of the tasks of degree buckets b_lo .. b (a contiguous range of the
 // task list: tasks are ordered by decreasing rating count) with one shape.
@@ -319,7 +319,11 @@ hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::Sw
   x.persistent = cfg.persistent;
   x.occupancy = cfg.persistent ? 0 : cfg.occupancy;
   x.task_counter = h->counters.as<int>() + side * kBuckets + b;
-  if (cfg.solver == CUMF_SOLVER_DUAL) return dual_launch(x, cfg.variant, stream);
+  if (cfg.solver == CUMF_SOLVER_DUAL) {
+    // staged 32-rating chunks: two up to 63 ratings, three up to 95
+    x.dual_chunks = kBucketEdge[b + 1] <= 64 ? 2 : 3;
+    return dual_launch(x, cfg.variant, stream);
+  }
   if (cfg.solver >= 0) x.solver = cfg.solver;
   return task_launch(x, cfg.variant, stream);
 }
@@ -344,9 +348,7 @@ int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs&
     }
     uniform = uniform && c.persistent == 0 && c.variant == v0 && sv == s0 && c.occupancy == o0;
   }
-  if (uniform && v0 != 0 && s0 == CUMF_SOLVER_DUAL)
-    return timed(h, Timer::TASK, [&] { return dual_launch(a, v0, h->stream); });
-  if (uniform && v0 != 0) {
+  if (uniform && v0 != 0 && s0 != CUMF_SOLVER_DUAL) {
     cumf::SweepArgs x = a;
     x.solver = s0;
     x.occupancy = o0;
@@ -362,11 +364,16 @@ int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs&
     return x.persistent == 0 && y.persistent == 0 && x.variant == y.variant && x.solver == y.solver &&
            x.occupancy == y.occupancy;
   };
+  // (the dual form stages two chunks for rows of up to 63 ratings, three
+  // above: those buckets do not share a launch)
+  auto dual_ch = [&](int b) { return kBucketEdge[b + 1] <= 64 ? 2 : 3; };
   int grp_hi[kBuckets], grp_lo[kBuckets], ng = 0;
   for (int b = kBuckets - 1; b >= 0; --b) {
     if (p.bucket_t1[b] <= p.bucket_t0[b]) continue;
     int lo = b;
-    while (lo > 0 && (p.bucket_t1[lo - 1] <= p.bucket_t0[lo - 1] || same(h->tiles[side][lo - 1], h->tiles[side][b])))
+    while (lo > 0 && (p.bucket_t1[lo - 1] <= p.bucket_t0[lo - 1] ||
+                      (same(h->tiles[side][lo - 1], h->tiles[side][b]) &&
+                       (h->tiles[side][b].solver != CUMF_SOLVER_DUAL || dual_ch(lo - 1) == dual_ch(b)))))
       --lo;
     while (p.bucket_t1[lo] <= p.bucket_t0[lo]) ++lo;  // the range ends at a non-empty bucket
     grp_hi[ng] = b;

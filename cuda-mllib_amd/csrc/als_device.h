@@ -1460,12 +1460,13 @@ __device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepA
 // mu = lambda n, the push-through identity
 //     (Y^T Y + mu I)^{-1} Y^T r = Y^T (Y Y^T + mu I)^{-1} r
 // gives the same x from an n x n system: z solves (Y Y^T + mu I) z = r and
-// x = Y^T z.  For n <= kDualMaxRatings (63) that system has at most four
-// 16-column blocks against NB for the f x f one, so the Cholesky costs a
-// fraction of the primal one (the primal Gram's MFMA work is about the
-// same as the dual's).
+// x = Y^T z.  For n <= kDualMaxRatings (95) that system has at most six
+// 16-column blocks against NB for the f x f one (seven at f = 100), so the
+// Cholesky costs a fraction of the primal one (the primal Gram's MFMA work
+// is about the same as the dual's).
 //   * Y is gathered as in gram_range_bf16: chunks of 32 rows, NB tiles of
-//     32 rows x 16 bf16, both chunks at once (rows past n are zeros).
+//     32 rows x 16 bf16, all CH chunks (2 for rows of up to 63 ratings, 3
+//     up to 95) at once (rows past n are zeros).
 //   * K = Y Y^T on v_mfma_f32_16x16x32_bf16: for k-step s and row block I,
 //     lane (kg, i) supplies row 16I + i, columns 32s + 8kg .. +7 (one
 //     16-byte LDS read; columns past the staged tiles read a zero block).
@@ -1478,22 +1479,30 @@ __device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepA
 //     Cholesky reused their LDS) and lane l forms columns 2l, 2l+1 from
 //     them, rating by rating.
 // ---------------------------------------------------------------------------
-constexpr int kDualMaxRatings = 2 * KBB - 1;
+constexpr int kDualMaxRatings = kDualMaxRatingsHost;  // three staged chunks
+static_assert(kDualMaxRatings == 3 * KBB - 1, "the dual form stages at most three chunks");
 
-template <int NB>
+// Row blocks (of 16) of the largest dual system CH staged chunks allow:
+// n <= 32 CH - 1 ratings plus r's column.
+__host__ __device__ constexpr int dual_max_blocks(int CH) { return (32 * CH - 1 + 16) / 16; }
+// LDS floats of the dual form: the CH staged chunks (the Cholesky's space
+// once the Gram is formed), then z, the ratings and a zero block.
+template <int NB, int CH>
+__host__ __device__ constexpr int dual_zs_offset() {
+  return cmax(CH * NB * 256, chol_lds_floats(dual_max_blocks(CH)));
+}
+template <int NB, int CH>
 __host__ __device__ constexpr int dual_lds_floats() {
-  // the two staged chunks, then z (64), the ratings (64) and a zero block (4)
-  return cmax(2 * NB * 256, chol_lds_floats(4)) + 64 + 64 + 4;
+  return dual_zs_offset<NB, CH>() + 96 + 96 + 4;
 }
 
-template <int NB, int NBD>
-__device__ __forceinline__ bool dual_blocks(const SweepArgs& a, int n, float* lds, int lane) {
+template <int NB, int CH, int NBD>
+__device__ __forceinline__ bool dual_blocks(const SweepArgs& a, int n, float* lds, float* zs, const float* rs,
+                                            const __bf16* zero, int lane) {
   constexpr int TB = KBB * 16, CB = NB * TB;  // bf16 per staged tile / chunk
   constexpr int S = (NB + 1) / 2;             // 32-column k-steps over the staged tiles
+  static_assert(NBD <= dual_max_blocks(CH), "rows past the staged chunks");
   const __bf16* yb = reinterpret_cast<const __bf16*>(lds);
-  float* zs = lds + cmax(2 * NB * 256, chol_lds_floats(4));
-  const float* rs = zs + 64;
-  const __bf16* zero = reinterpret_cast<const __bf16*>(zs + 128);
   const int kg = lane >> 4, i = lane & 15;
   f4 acc[num_tiles(NBD)];
 #pragma unroll
@@ -1526,38 +1535,39 @@ __device__ __forceinline__ bool dual_blocks(const SweepArgs& a, int n, float* ld
   return chol_solve<NBD>(acc, n, 16 * NBD - 1, lds, zs, 16 * NBD, lane);
 }
 
-template <int NB>
+template <int NB, int CH>
 __device__ __forceinline__ void dual_solve_row(const SweepArgs& a, const Task& t, float* lds, int lane) {
   constexpr int TB = KBB * 16, CB = NB * TB;
   float* xrow = a.x + (int64_t)t.row * a.ldf;
   __bf16* xb = reinterpret_cast<__bf16*>(a.x_b) + (int64_t)t.row * a.ldb;
   const int n = t.count;
-  if (n <= 0 || n > kDualMaxRatings) {  // (the host launches only rows of at most 63 ratings)
+  if (n <= 0 || n > KBB * CH - 1) {  // (the host launches only rows that fit)
     for (int i = lane; i < a.ldf; i += 64) xrow[i] = 0.0f;
     for (int i = lane; i < a.ldb; i += 64) xb[i] = (__bf16)0.0f;
-    if (n > kDualMaxRatings && lane == 0) atomicAdd(a.fail_counter, 1);
+    if (n > 0 && lane == 0) atomicAdd(a.fail_counter, 1);
     return;
   }
   const __bf16* theta_b = reinterpret_cast<const __bf16*>(a.theta_b);
   __bf16* yb = reinterpret_cast<__bf16*>(lds);
-  float* zs = lds + cmax(2 * NB * 256, chol_lds_floats(4));
-  float* rs = zs + 64;
-  // Gather both chunks: lane l moves row l/2 of the chunk, 16 bytes of each
+  float* zs = lds + dual_zs_offset<NB, CH>();
+  float* rs = zs + 96;
+  const __bf16* zero = reinterpret_cast<const __bf16*>(rs + 96);
+  // Gather the CH chunks: lane l moves row l/2 of a chunk, 16 bytes of each
   // tile (rows past n and slots past ldb come from kZeroRow).  Done twice:
   // for the Gram, and again after the Cholesky (which reuses the space) for
   // x = Y^T z.
   const int sr = lane >> 1, sh = lane & 1;
   const bool last_in = 16 * (NB - 1) + 8 * sh < (int)a.ldb;
   const __bf16* zb = reinterpret_cast<const __bf16*>(kZeroRow);
-  const __bf16* rp[2];
+  const __bf16* rp[CH];
 #pragma unroll
-  for (int ch = 0; ch < 2; ++ch) {
+  for (int ch = 0; ch < CH; ++ch) {
     const int k = ch * KBB + sr;
     rp[ch] = k < n ? theta_b + (int64_t)a.col[t.start + k] * a.ldb + 8 * sh : zb;
   }
   auto gather = [&] {
 #pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
+    for (int ch = 0; ch < CH; ++ch) {
       __bf16* dst = yb + ch * CB;
 #pragma unroll
       for (int I = 0; I < NB - 1; ++I)
@@ -1567,16 +1577,19 @@ __device__ __forceinline__ void dual_solve_row(const SweepArgs& a, const Task& t
     }
   };
   gather();
-  rs[lane] = lane < n ? a.val[t.start + lane] : 0.0f;
-  if (lane < 4) zs[128 + lane] = 0.0f;  // the zero block
+  for (int i = lane; i < 96; i += 64) rs[i] = i < n ? a.val[t.start + i] : 0.0f;
+  if (lane < 4) rs[96 + lane] = 0.0f;  // the zero block
   wait_vm_all();
   lds_order();
   bool ok;
   const int nbd = (n + 16) >> 4;  // blocks of the n x n system plus r's column
-  if (nbd == 1) ok = dual_blocks<NB, 1>(a, n, lds, lane);
-  else if (nbd == 2) ok = dual_blocks<NB, 2>(a, n, lds, lane);
-  else if (nbd == 3) ok = dual_blocks<NB, 3>(a, n, lds, lane);
-  else ok = dual_blocks<NB, 4>(a, n, lds, lane);
+  if (nbd == 1) ok = dual_blocks<NB, CH, 1>(a, n, lds, zs, rs, zero, lane);
+  else if (nbd == 2) ok = dual_blocks<NB, CH, 2>(a, n, lds, zs, rs, zero, lane);
+  else if (nbd == 3) ok = dual_blocks<NB, CH, 3>(a, n, lds, zs, rs, zero, lane);
+  else if constexpr (CH == 2) ok = dual_blocks<NB, CH, 4>(a, n, lds, zs, rs, zero, lane);
+  else if (nbd == 4) ok = dual_blocks<NB, CH, 4>(a, n, lds, zs, rs, zero, lane);
+  else if (nbd == 5) ok = dual_blocks<NB, CH, 5>(a, n, lds, zs, rs, zero, lane);
+  else ok = dual_blocks<NB, CH, 6>(a, n, lds, zs, rs, zero, lane);
   lds_order();
   gather();
   wait_vm_all();

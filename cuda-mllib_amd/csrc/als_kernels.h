@@ -71,7 +71,13 @@ struct SweepArgs {
   // occupancy > 0 (one-task-per-group launches): at most that many
   // wavefronts per CU, enforced through the LDS each group requests.
   int occupancy;
+  // Dual-form launches: 32-rating chunks staged per row, 2 (rows of at
+  // most 63 ratings) or 3 (at most 95).
+  int dual_chunks;
 };
+
+// Longest row the dual form takes (three staged chunks of 32 ratings).
+constexpr int kDualMaxRatingsHost = 95;
 
 // LDS of one CU, and the range of occupancy caps a launch accepts.
 constexpr int kLdsBytesPerCU = 160 * 1024;

@@ -181,7 +181,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
 // Dual form of the tasks of one launch, each of at most kDualMaxRatings
 // ratings (BF16 factors).  The last chunk of a split row can be that short:
 // such a task still forms its partial Gram.
-template <int NB>
+template <int NB, int CH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_dual_kernel(SweepArgs a) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x;
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
       for (int r = 0; r < 4; ++r) dst[(i * 4 + r) * 64 + lane] = acc[i][r];
     return;
   }
-  dual_solve_row<NB>(a, t, lds, lane);
+  dual_solve_row<NB, CH>(a, t, lds, lane);
 }
 #endif
 
@@ -237,8 +237,15 @@ static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which) {
       // which == 4: the dual form (BF16 factors only)
       if (!bf) return hipErrorInvalidValue;
       if (a.ntasks > 0) {
-        const size_t shd = sizeof(float) * (size_t)cmax(dual_lds_floats<NB>(), lds_floats_nb(NB, true));
-        hipLaunchKernelGGL((als_dual_kernel<NB>), dim3((unsigned)a.ntasks), dim3(64), shd, s, a);
+        if (a.dual_chunks == 2) {
+          const size_t shd = sizeof(float) * (size_t)cmax(dual_lds_floats<NB, 2>(), lds_floats_nb(NB, true));
+          hipLaunchKernelGGL((als_dual_kernel<NB, 2>), dim3((unsigned)a.ntasks), dim3(64), shd, s, a);
+        } else if (a.dual_chunks == 3) {
+          const size_t shd = sizeof(float) * (size_t)cmax(dual_lds_floats<NB, 3>(), lds_floats_nb(NB, true));
+          hipLaunchKernelGGL((als_dual_kernel<NB, 3>), dim3((unsigned)a.ntasks), dim3(64), shd, s, a);
+        } else {
+          return hipErrorInvalidValue;
+        }
       }
 #endif
     }

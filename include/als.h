@@ -84,8 +84,8 @@ void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, f
                                    current factor; rows whose relative residual
                                    stays above cg_tol fall back to the Cholesky
                                    solve */
-#define CUMF_SOLVER_DUAL 2      /* tile tables only, degree buckets 0 and 1 (at
-                                   most 63 ratings) of a CUMF_PREC_BF16_GRAM
+#define CUMF_SOLVER_DUAL 2      /* tile tables only, degree buckets 0, 1 and 2 (at
+                                   most 95 ratings) of a CUMF_PREC_BF16_GRAM
                                    handle: the same x from the n x n system
                                    (Y Y^T + lambda n I) z = r, x = Y^T z, with
                                    Y the row's n gathered factor rows (FP32
@@ -248,7 +248,8 @@ int cumf_csr_to_csc_device(hipStream_t stream, int64_t m, int64_t n, int64_t nnz
  * reference, which launches one block per row).
  *
  * Tasks (rows, or split chunks of long rows) fall into 8 degree buckets by
- * rating count ([0,16), [16,64), ..., [16384,65536), [65536,...)).  A tile
+ * rating count ([0,32), [32,64), [64,96), [96,256), [256,1024),
+ * [1024,4096), [4096,16384), [16384,...)).  A tile
  * table gives, per side (0 = X, 1 = Theta): the split size, the register
  * budget (1 or 2 wavefronts per SIMD) of the split-row finish and of the CG
  * fallback kernels, and per bucket the register budget, whether the bucket
