@@ -111,9 +111,9 @@ def cpu_baseline(ds, f, lam, theta, X, target_s):
 # profiled configuration (profiles/als_task_kernel_netflix480k_f100_counters.md).
 LIMITER = {
     ("netflix_480k_17770_99m", "bf16-cg-tuned", 100):
-        "X launch: issue and latency of the per-row Cholesky (matrix pipe 43% busy, 44% of wave time "
-        "waiting to issue, ~3,000 VALU instructions per row); Theta launch: BF16 gathers beyond L2 "
-        "(44% of wave time waiting on memory)",
+        "X half-sweep (dual form up to 95 ratings, Cholesky above): latency of gathers and of the "
+        "per-row solves (33% of wave time waiting on memory, 34% waiting to issue, matrix pipe 29% busy); "
+        "Theta half-sweep: BF16 gathers beyond L2 (44% of wave time waiting on memory)",
 }
 
 
