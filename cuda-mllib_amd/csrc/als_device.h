@@ -1595,8 +1595,10 @@ __device__ __forceinline__ void dual_solve_row(const SweepArgs& a, const Task& t
   wait_vm_all();
   lds_order();
   // x = Y^T z from the staged rows (chol_solve left z = 0 if it failed).
-  // Lane l forms columns c0 = 2l + 128 p and c0 + 1 (tile c0 / 16).
-  for (int c0 = 2 * lane; c0 < (int)a.ldb; c0 += 128) {
+  // Lane l forms columns c0 = 2l + 128 p and c0 + 1 (tile c0 / 16) of the
+  // staged width; the BF16 row's padding past it is zero.
+  for (int i = 16 * NB + lane; i < (int)a.ldb; i += 64) xb[i] = (__bf16)0.0f;
+  for (int c0 = 2 * lane; c0 < 16 * NB; c0 += 128) {
     const __bf16* col = yb + (c0 >> 4) * TB + (c0 & 15);
     float x0 = 0.0f, x1 = 0.0f;
 #pragma unroll 8
@@ -1610,8 +1612,8 @@ __device__ __forceinline__ void dual_solve_row(const SweepArgs& a, const Task& t
     x1 = c0 + 1 < a.f ? x1 : 0.0f;
     if (c0 < (int)a.ldf) xrow[c0] = x0;
     if (c0 + 1 < (int)a.ldf) xrow[c0 + 1] = x1;
-    xb[c0] = (__bf16)x0;
-    xb[c0 + 1] = (__bf16)x1;
+    if (c0 < (int)a.ldb) xb[c0] = (__bf16)x0;
+    if (c0 + 1 < (int)a.ldb) xb[c0 + 1] = (__bf16)x1;
   }
   if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
 }
