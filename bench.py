@@ -492,7 +492,9 @@ def main():
         "half_sweep_ms": res["sweep_ms"],
         "learned": learned,
         "batching": res["tuning"],
-        "roofline": {"kernel": "als_task_kernel (fused gram+solve), per half-sweep", "bound": "mfma",
+        "roofline": {"kernel": ("als_task_kernel (fused gram+solve)" if f <= 140 else
+                                "als_large_kernel + als_large_dual_kernel (work-group per row; dual form "
+                                "for rows of at most 95 ratings)") + ", per half-sweep", "bound": "mfma",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "avg_launch_ms": task_ms,

@@ -13,6 +13,7 @@
 // order by the finish kernel.  Tasks are ordered by decreasing length so
 // the longest start first (longest-processing-time-first scheduling).
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdio>
 #include <cmath>
@@ -50,6 +51,11 @@ int64_t round_up4(int64_t f) { return (f + 3) & ~int64_t(3); }
 constexpr int kBuckets = 8;
 constexpr int64_t kBucketEdge[kBuckets + 1] = {0, 32, 64, 96, 256, 1024, 4096, 16384, INT64_MAX};
 constexpr int kAuxStreams = 3;
+
+// Above kMaxFusedF, rows of at most kDualMaxRatingsHost ratings (degree
+// buckets 0 .. 2) are solved in the dual form unless a test switched it off
+// (cumf_debug_large_dual).
+std::atomic<int> g_large_dual{1};
 
 // Launch shape of one bucket: register budget (wavefronts per SIMD the
 // kernel was compiled for, 1 or 2), persistent grid (0: one work-group per
@@ -101,6 +107,7 @@ int plan_tasks(const int32_t* row_ptr, int64_t m, int split_, int f, std::vector
 // Wavefront task list of one rating matrix side on the device.
 struct Plan {
   int64_t rows = 0, nnz = 0, ntasks = 0, nheavy = 0, nslots = 0;
+  int64_t light0 = 0;  // first task of at most kDualMaxRatingsHost ratings (tasks light0 .. ntasks-1)
   int split = 0;
   int64_t bucket_t0[kBuckets] = {}, bucket_t1[kBuckets] = {};
   DevBuf tasks, heavy, partial;
@@ -121,6 +128,9 @@ struct Plan {
       if (bucket_t1[b] == 0) bucket_t0[b] = i;
       bucket_t1[b] = i + 1;
     }
+    light0 = ntasks;
+    for (int b = 0; b < kBuckets && kBucketEdge[b + 1] <= cumf::kDualMaxRatingsHost + 1; ++b)
+      if (bucket_t1[b] > bucket_t0[b]) light0 = std::min(light0, bucket_t0[b]);
     nheavy = (int64_t)hv.size();
     nslots = slots;
     CUMF_TRY(tasks.alloc(sizeof(cumf::Task) * t.size()));
@@ -341,6 +351,30 @@ hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::Sw
   return task_launch(x, cfg.variant, stream);
 }
 
+// The half-sweep above kMaxFusedF over a plan's whole-row tasks: tasks
+// 0 .. nh-1 on the work-group-per-row kernel, the nl tasks after them (at
+// most kDualMaxRatingsHost ratings each) in the dual form, side by side
+// when an auxiliary stream is given (forked from s and joined back).
+hipError_t large_launches(const cumf::SweepArgs& a, int64_t nh, int64_t nl, hipStream_t s, hipStream_t aux = nullptr,
+                          hipEvent_t fork = nullptr, hipEvent_t join = nullptr) {
+  cumf::SweepArgs d = a;
+  d.tasks = a.tasks + nh;
+  const bool both = aux && nh > 0 && nl > 0;
+  hipError_t e = hipSuccess;
+  if (both) {
+    e = hipEventRecord(fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
+  }
+  if (e == hipSuccess && nh > 0) e = cumf::launch_large_kernel(a, nh, s);
+  if (e == hipSuccess && nl > 0) e = cumf::launch_large_dual_kernel(d, nl, both ? aux : s);
+  if (e == hipSuccess && both) {
+    e = hipEventRecord(join, aux);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
+  }
+  return e;
+}
+int64_t large_dual_tasks(const Plan& p) { return g_large_dual.load() ? p.ntasks - p.light0 : 0; }
+
 // All tasks of a half-sweep: one launch, or with a tile table one launch
 // per non-empty degree bucket (heaviest first), persistent ones after their
 // counters are cleared.
@@ -509,7 +543,20 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
   cumf::SweepArgs a{};
   CUMF_TRY(fill_args(h, p, rowp, col, val, theta, x, theta_b, x_b, trace, a));
   if (h->f > kMaxFusedF) {
-    CUMF_TRY(timed(h, Timer::TASK, [&] { return cumf::launch_large_kernel(a, p.rows, h->stream); }));
+    const int64_t nl = large_dual_tasks(p), nh = p.ntasks - nl;
+    hipEvent_t s0 = nullptr, s1 = nullptr;
+    if (h->opt.timing && !h->capturing) {
+      s0 = h->timer.get();
+      s1 = h->timer.get();
+      if (s0) (void)hipEventRecord(s0, h->stream);
+    }
+    CUMF_TRY(timed(h, Timer::TASK,
+                   [&] { return large_launches(a, nh, nl, h->stream, h->aux[0], h->fork_ev, h->join_ev[0]); },
+                   std::max(1, (nh > 0) + (nl > 0))));
+    if (s0 && s1) {
+      (void)hipEventRecord(s1, h->stream);
+      h->timer.pending.push_back({s0, s1, Timer::SWEEP, 1});
+    }
     h->gram_flops += p.nnz * (int64_t)h->f * (h->f + 1);
     return CUMF_OK;
   }
@@ -1034,6 +1081,8 @@ int cumf_reset_stats(cumf_handle* h) {
 
 hipStream_t cumf_get_stream(cumf_handle* h) { return h ? h->stream : nullptr; }
 
+int cumf_debug_large_dual(int on) { return g_large_dual.exchange(on ? 1 : 0); }
+
 int cumf_debug_gram_dim(int f) { return (f >= 1 && f <= kMaxFusedF) ? cumf::dump_dim(f) : -1; }
 
 int cumf_debug_plan(const int32_t* row_ptr, int64_t m, int split, int f, int32_t* tasks, int64_t task_cap,
@@ -1373,8 +1422,8 @@ int cumf_load_tile_table(cumf_handle* h, const char* path) {
       // fallback list and kernel); Cholesky is always available.
       if (nf == 6) sol = -1;
       if (nf <= 7) occ = 0;
-      // The dual form needs BF16 factors and rows of at most 63 ratings
-      // (buckets 0 and 1), one task per group.
+      // The dual form needs BF16 factors and rows of at most 95 ratings
+      // (buckets 0 to 2), one task per group.
       if (side < 0 || side > 1 || b < 0 || b >= kBuckets || (v != 1 && v != 2) || pers < 0 || pers > (1 << 20) ||
           sol < -1 || sol > 2 || (sol == CUMF_SOLVER_CG && h->opt.solver != CUMF_SOLVER_CG) ||
           (sol == CUMF_SOLVER_DUAL && (h->opt.precision == CUMF_PREC_FP32 || b >= kDualBuckets || pers > 0)) ||
@@ -1828,7 +1877,8 @@ int half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const int32_
     CUMF_HIP_TRY(cumf::launch_compat(c, m, stream));
   } else if (f > kMaxFusedF) {
     CUMF_HIP_TRY(cumf::prepare_large_kernel(f));
-    CUMF_HIP_TRY(cumf::launch_large_kernel(a, m, stream));
+    const int64_t nl = large_dual_tasks(*plan);
+    CUMF_HIP_TRY(large_launches(a, plan->ntasks - nl, nl, stream));
   } else {
     CUMF_HIP_TRY(task_launch(a, default_variant(f), stream));
     CUMF_HIP_TRY(finish_launch(a, default_variant(f), stream));

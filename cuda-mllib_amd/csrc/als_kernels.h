@@ -113,7 +113,7 @@ hipError_t launch_fallback_reset(int* count, int n, hipStream_t s);
 // Launchers of the sweep kernels, per register budget (w1, w2) and solver
 // family (chol: Cholesky task/finish kernels and the CG path's fallback;
 // cg: CG task/finish kernels; chol also holds the dual-form task kernel
-// for rows of at most 63 ratings, BF16 factors only).
+// for rows of at most 95 ratings, BF16 factors only).
 #define CUMF_SWEEP_LAUNCHERS                                                \
   namespace chol {                                                          \
   hipError_t launch_task_kernel(const SweepArgs& a, hipStream_t s);         \
@@ -129,12 +129,16 @@ namespace w1 { CUMF_SWEEP_LAUNCHERS }
 namespace w2 { CUMF_SWEEP_LAUNCHERS }
 #undef CUMF_SWEEP_LAUNCHERS
 // Half-sweep for 140 < f <= 256 (FP32 Gram + FP32 Cholesky, one 256-thread
-// work-group per row over row_ptr[0 .. rows]); large_lds_bytes(f) of LDS.
+// work-group per row: the whole-row tasks a.tasks[0 .. rows), or rows
+// 0 .. rows-1 of row_ptr when a.tasks is null); large_lds_bytes(f) of LDS.
 // prepare_large_kernel(f) raises the kernel's dynamic LDS limit to
 // large_lds_bytes(f); call it outside any stream capture before launching.
 hipError_t launch_large_kernel(const SweepArgs& a, int64_t rows, hipStream_t s);
 hipError_t prepare_large_kernel(int f);
 size_t large_lds_bytes(int f);
+// The dual form for 140 < f <= 256: one wavefront per whole-row task
+// a.tasks[0 .. ntasks), each of at most kDualMaxRatingsHost ratings (FP32).
+hipError_t launch_large_dual_kernel(const SweepArgs& a, int64_t ntasks, hipStream_t s);
 hipError_t launch_dump_kernel(const SweepArgs& a, const int32_t* rows, int64_t nrows, float* out,
                               hipStream_t s);
 hipError_t launch_rmse(const int32_t* u, const int32_t* v, const float* r, int64_t cnt, const float* X,

@@ -15,16 +15,36 @@
 //     that gathers the last tile), double-buffered with one barrier per
 //     chunk.
 //   * The tiles then go to LDS (1 KB each, in accumulator order), and a
-//     blocked right-looking Cholesky runs on them with three barriers per
-//     16-column block K: wavefront 0 factors the diagonal tile into
-//     W_K = U_KK^{-T} (the column-per-lane elimination of the fused
-//     kernels); the four wavefronts form the panel U_KJ = W_K A_KJ and the
-//     trailing update A_IJ -= U_KI^T U_KJ on MFMA, tile by tile.  The
-//     augmented column ends as z = U^{-T} b.
+//     blocked right-looking Cholesky runs on them with two barriers per
+//     16-column block K: the four wavefronts form the panel
+//     U_KJ = W_K A_KJ on MFMA; then wavefront 0 updates the next diagonal
+//     tile and factors it into W_{K+1} = U_{K+1,K+1}^{-T} (the
+//     column-per-lane elimination of the fused kernels) while wavefronts
+//     1-3 apply the rest of the trailing update A_IJ -= U_KI^T U_KJ on
+//     MFMA, tile by tile.  The augmented column ends as z = U^{-T} b.
 //   * Blocked back substitution x_K = W_K^T (z_K - sum_{J>K} U_KJ x_J) in
 //     wavefront 0.
 // Rows without ratings get x = 0; a pivot that is not positive and finite
 // counts a failed row and gives x = 0.
+//
+// Rows of at most 95 ratings take the dual form instead (one wavefront per
+// row, als_large_dual_kernel): with Y the n x f matrix of the row's factor
+// rows and mu = lambda n, (Y^T Y + mu I)^{-1} Y^T r = Y^T (Y Y^T + mu I)^{-1} r,
+// so z solves the n x n system (Y Y^T + mu I) z = r and x = Y^T z:
+//   * K = Y Y^T in FP32 on v_mfma_f32_16x16x4_f32, straight from the factor
+//     rows: for each 16-column slab, lane (g, c) loads columns 4g .. 4g+3 of
+//     row 16I + c (16 bytes; the four lanes g of a row read 64 contiguous
+//     bytes) for every row block I, and the four columns feed four MFMA
+//     steps in which the same register is the A operand of block I and the
+//     B operand of block J = I.  The next slab's loads are issued before the
+//     current slab's MFMAs.
+//   * r goes into column 16 NBD - 1 (past every live pivot), mu onto the
+//     first n diagonal entries, and the register-tile Cholesky of the fused
+//     kernels factors the NBD = (n + 16) / 16 <= 6 blocks and writes z to LDS.
+//   * x = Y^T z in FP32: lane l forms columns 4l .. 4l+3, reading the n rows
+//     again (one 16-byte load per lane and rating).
+// At f = 256 that is at most 21 tiles of K against the 153 tiles of the
+// primal Gram and a 6-block Cholesky against 17 blocks.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -195,10 +215,11 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
   constexpr int NT = num_tiles(NB);
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  const int64_t u = blockIdx.x;
+  // Whole rows: task blockIdx.x of a.tasks, or row blockIdx.x without a list.
+  const int64_t u = a.tasks ? (int64_t)a.tasks[blockIdx.x].row : (int64_t)blockIdx.x;
   const int f = a.f;
-  const int64_t s0 = a.row_ptr[u];
-  const int cnt = a.row_ptr[u + 1] - a.row_ptr[u];
+  const int64_t s0 = a.tasks ? (int64_t)a.tasks[blockIdx.x].start : (int64_t)a.row_ptr[u];
+  const int cnt = a.tasks ? a.tasks[blockIdx.x].count : a.row_ptr[u + 1] - a.row_ptr[u];
   float* xrow = a.x + u * a.ldf;
   if (cnt <= 0) {
     for (int i = threadIdx.x; i < (int)a.ldf; i += kLT) xrow[i] = 0.0f;
@@ -224,12 +245,21 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
   __syncthreads();  // all tiles in their slots
 
   auto slot = [&](int I, int J) { return lds + tile_index(I, J, NB) * 256; };
-  for (int K = 0; K < NB; ++K) {
-    if (16 * K >= f) break;  // only the last block can hold no pivot of the f x f system
-    const bool last = K + 1 == NB;
-    if (wv == 0) diag_tile<NB>(slot(K, K), last ? wlast : slot(K, K), tsc, ident, f - 16 * K, last, lane);
-    __syncthreads();  // W_K is in LDS
-    if (last) break;
+  // A_IJ -= U_KI^T U_KJ on one tile.
+  auto trailing_tile = [&](int K, int I, int J) {
+    const f4 uv = *reinterpret_cast<const f4*>(slot(K, I) + 4 * lane);
+    const f4 v = *reinterpret_cast<const f4*>(slot(K, J) + 4 * lane);
+    float* p = slot(I, J);
+    f4 d = *reinterpret_cast<const f4*>(p + 4 * lane);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) d = __builtin_amdgcn_mfma_f32_16x16x4f32(-uv[s], v[s], d, 0, 0, 0);
+    *reinterpret_cast<f4*>(p + 4 * lane) = d;
+  };
+  // Block 0's diagonal tile (NB >= 10 here: block 0 is neither the last
+  // nor short of pivots).
+  if (wv == 0) diag_tile<NB>(slot(0, 0), slot(0, 0), tsc, ident, f, false, lane);
+  __syncthreads();  // W_0 is in LDS
+  for (int K = 0; K + 1 < NB; ++K) {
     {
       // Panel: U_KJ = W_K A_KJ; A operand W_K[c][4g+s] from the column-major W.
       const float* w = slot(K, K);
@@ -246,24 +276,26 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
       }
     }
     __syncthreads();  // the panel is final
-    {
-      // Trailing update of tiles (I, J), K < I <= J, dealt round-robin.
-      int q = 0;
-      for (int I = K + 1; I < NB; ++I) {
-        const f4 uv = *reinterpret_cast<const f4*>(slot(K, I) + 4 * lane);
-        const f4 nu = {-uv[0], -uv[1], -uv[2], -uv[3]};
-        for (int J = I; J < NB; ++J, ++q) {
-          if ((q & 3) != wv) continue;
-          float* p = slot(I, J);
-          const f4 v = *reinterpret_cast<const f4*>(slot(K, J) + 4 * lane);
-          f4 d = *reinterpret_cast<const f4*>(p + 4 * lane);
-#pragma unroll
-          for (int s = 0; s < 4; ++s) d = __builtin_amdgcn_mfma_f32_16x16x4f32(nu[s], v[s], d, 0, 0, 0);
-          *reinterpret_cast<f4*>(p + 4 * lane) = d;
-        }
+    // Lookahead: wavefront 0 updates the next diagonal tile and factors it
+    // while wavefronts 1-3 apply the rest of block K's trailing update
+    // (tiles (I, J), K < I <= J, dealt round-robin).  Only the last block
+    // can hold no pivot of the f x f system; its diagonal tile is then not
+    // factored (its x is zero).
+    const int K1 = K + 1;
+    if (wv == 0) {
+      trailing_tile(K, K1, K1);
+      lds_order();
+      if (16 * K1 < f) {
+        const bool last = K1 + 1 == NB;
+        diag_tile<NB>(slot(K1, K1), last ? wlast : slot(K1, K1), tsc, ident, f - 16 * K1, last, lane);
       }
+    } else {
+      int q = 0;
+      for (int I = K1; I < NB; ++I)
+        for (int J = (I == K1 ? K1 + 1 : I); J < NB; ++J, ++q)
+          if (q % 3 == wv - 1) trailing_tile(K, I, J);
     }
-    __syncthreads();  // the trailing tiles are updated
+    __syncthreads();  // block K's trailing update and W_{K+1} are in LDS
   }
   if (wv != 0) return;
 
@@ -305,6 +337,129 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
   for (int i = lane; i < (int)a.ldf; i += 64) xrow[i] = (ok && i < f) ? xs[i] : 0.0f;
 }
 
+// Dual form: row blocks of the largest system (95 ratings plus r's column),
+// and the LDS floats of a row: the Cholesky's space for that many blocks,
+// then z, the ratings and the ratings' column indices (96 each).
+constexpr int kLargeDualBlocks = (kDualMaxRatings + 16) / 16;
+static_assert(kLargeDualBlocks == 6, "six row blocks");
+__host__ __device__ constexpr int large_dual_lds_floats() { return chol_lds_floats(kLargeDualBlocks) + 3 * 96; }
+
+// K = Y Y^T for the n rows listed in cs (n + 1 <= 16 NBD), r and the ridge
+// added, factored; z into zs.  False if a pivot was not positive and
+// finite.
+template <int NBD>
+__device__ __forceinline__ bool large_dual_blocks(const SweepArgs& a, int n, const int* cs, const float* rs,
+                                                  float* lds, float* zs, int lane) {
+  const int g = lane >> 4, c = lane & 15;
+  const int ldf = (int)a.ldf;
+  const int S = (ldf + 15) >> 4;  // 16-column slabs; columns past ldf are zero
+  const float* rp[NBD];
+#pragma unroll
+  for (int I = 0; I < NBD; ++I) {
+    const int r = 16 * I + c;
+    rp[I] = r < n ? a.theta + (int64_t)cs[r] * ldf : kZeroRow;  // (kZeroRow holds 256 >= ldf floats)
+  }
+  auto load = [&](int s, f4 (&v)[NBD]) {
+    const int k = 16 * s + 4 * g;
+#pragma unroll
+    for (int I = 0; I < NBD; ++I)
+      v[I] = k < ldf ? *reinterpret_cast<const f4*>(rp[I] + k) : f4{0.f, 0.f, 0.f, 0.f};
+  };
+  f4 acc[num_tiles(NBD)];
+#pragma unroll
+  for (int t = 0; t < num_tiles(NBD); ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  // Four MFMA steps per slab: in step j, lane (g, c) supplies column
+  // 16 s + 4 g + j of row 16 I + c, so every column enters each tile once.
+  auto mfmas = [&](const f4 (&v)[NBD]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int I = 0; I < NBD; ++I)
+#pragma unroll
+        for (int J = I; J < NBD; ++J)
+          acc[tile_index(I, J, NBD)] =
+              __builtin_amdgcn_mfma_f32_16x16x4f32(v[I][j], v[J][j], acc[tile_index(I, J, NBD)], 0, 0, 0);
+  };
+  f4 va[NBD], vb[NBD];
+  load(0, va);
+  for (int s = 0; s < S; s += 2) {
+    if (s + 1 < S) load(s + 1, vb);
+    mfmas(va);
+    if (s + 1 < S) {
+      if (s + 2 < S) load(s + 2, va);
+      mfmas(vb);
+    }
+  }
+  // r into column 16 NBD - 1 of the last block column (K is zero there:
+  // rows past n are zero and 16 NBD - 1 >= n), mu on the diagonal.
+#pragma unroll
+  for (int I = 0; I < NBD; ++I)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c == 15) acc[tile_index(I, NBD - 1, NBD)][q] = rs[16 * I + 4 * g + q];
+  add_ridge<NBD>(acc, n, a.lambda * (float)n, lane);
+  return chol_solve<NBD>(acc, n, 16 * NBD - 1, lds, zs, 16 * NBD, lane);
+}
+
+// One wavefront per task (a whole row of at most 95 ratings).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void als_large_dual_kernel(SweepArgs a) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const Task t = a.tasks[blockIdx.x];
+  const int ldf = (int)a.ldf;
+  float* xrow = a.x + (int64_t)t.row * a.ldf;
+  const int n = t.count;
+  if (n <= 0 || n > kDualMaxRatings) {  // (the host launches only rows that fit)
+    for (int i = lane; i < ldf; i += 64) xrow[i] = 0.0f;
+    if (n > 0 && lane == 0) atomicAdd(a.fail_counter, 1);
+    return;
+  }
+  float* zs = lds + chol_lds_floats(kLargeDualBlocks);
+  float* rs = zs + 96;
+  int* cs = reinterpret_cast<int*>(rs + 96);
+  for (int i = lane; i < 96; i += 64) {
+    const bool in = i < n;
+    rs[i] = in ? a.val[t.start + i] : 0.0f;
+    cs[i] = in ? a.col[t.start + i] : 0;
+  }
+  lds_order();
+  const int nbd = (n + 16) >> 4;  // blocks of the n x n system plus r's column
+  bool ok;
+  if (nbd == 1) ok = large_dual_blocks<1>(a, n, cs, rs, lds, zs, lane);
+  else if (nbd == 2) ok = large_dual_blocks<2>(a, n, cs, rs, lds, zs, lane);
+  else if (nbd == 3) ok = large_dual_blocks<3>(a, n, cs, rs, lds, zs, lane);
+  else if (nbd == 4) ok = large_dual_blocks<4>(a, n, cs, rs, lds, zs, lane);
+  else if (nbd == 5) ok = large_dual_blocks<5>(a, n, cs, rs, lds, zs, lane);
+  else ok = large_dual_blocks<6>(a, n, cs, rs, lds, zs, lane);
+  lds_order();
+  // x = Y^T z (chol_solve left z = 0 if it failed): columns 4 lane .. +3.
+  const int k = 4 * lane;
+  if (k >= ldf) return;
+  f4 xv = {0.f, 0.f, 0.f, 0.f};
+  int i = 0;
+  for (; i + 4 <= n; i += 4) {
+    f4 y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[q] = *reinterpret_cast<const f4*>(a.theta + (int64_t)cs[i + q] * ldf + k);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float zq = zs[i + q];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) xv[e] = fmaf(zq, y[q][e], xv[e]);
+    }
+  }
+  for (; i < n; ++i) {
+    const f4 y = *reinterpret_cast<const f4*>(a.theta + (int64_t)cs[i] * ldf + k);
+    const float zq = zs[i];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) xv[e] = fmaf(zq, y[e], xv[e]);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) xv[e] = (ok && k + e < a.f) ? xv[e] : 0.0f;
+  *reinterpret_cast<f4*>(xrow + k) = xv;
+  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
+}
+
 template <int NB>
 hipError_t set_lds_limit() {
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&als_large_kernel<NB>),
@@ -332,6 +487,15 @@ hipError_t prepare_large_kernel(int f) {
     case 17: return set_lds_limit<17>();
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_large_dual_kernel(const SweepArgs& a, int64_t ntasks, hipStream_t s) {
+  if (a.f <= 0 || a.f > kLargeMaxF || a.ldf > kLargeMaxF || ntasks < 0 || (ntasks > 0 && !a.tasks))
+    return hipErrorInvalidValue;
+  if (ntasks == 0) return hipSuccess;
+  hipLaunchKernelGGL(als_large_dual_kernel, dim3((unsigned)ntasks), dim3(64),
+                     sizeof(float) * large_dual_lds_floats(), s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_large_kernel(const SweepArgs& a, int64_t rows, hipStream_t s) {

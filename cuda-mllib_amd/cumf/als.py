@@ -92,6 +92,7 @@ def lib():
         "cumf_debug_compat_system": ([P, C.c_int, P, C.c_int64, P], C.c_int),
         "cumf_debug_copy": ([C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "cumf_debug_force_host_alloc": ([C.c_int], C.c_int),
+        "cumf_debug_large_dual": ([C.c_int], C.c_int),
         "cumf_debug_cg_trace": ([P, C.c_int, P], C.c_int),
         "cumf_debug_features": ([P, C.c_int, P, C.c_int64, P], C.c_int),
         "cumf_set_mlp": ([P, P], C.c_int),

@@ -311,6 +311,10 @@ int cumf_debug_compat_system(cumf_handle* h, int side, const int32_t* rows, int6
  * the result, refinement rounds run, 1 if the row fell back to Cholesky.
  * out: host, rows x 4. */
 int cumf_debug_cg_trace(cumf_handle* h, int side, float* out);
+/* 0: above f = 140, every row takes the work-group-per-row primal kernel;
+ * 1 (the default): rows of at most 95 ratings take the dual form.  Process
+ * wide; returns the previous setting. */
+int cumf_debug_large_dual(int on);
 /* 1: mallocBest skips device memory and takes its mapped-host fallback
  * (also set by the environment variable CUMF_FORCE_HOST_ALLOC=1). */
 int cumf_debug_force_host_alloc(int on);

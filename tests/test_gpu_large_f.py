@@ -1,9 +1,12 @@
 """GPU checks of the half-sweep for 140 < f <= 256 (one work-group per row,
-FP32 Gram + FP32 Cholesky in LDS), through the C ABI, against the FP64
-oracle: factors within 1e-3 relative, per-row relative residual against
-the oracle's FP64 system <= 1e-4, the drop-in entry point (also with
-f % 4 != 0, the re-strided path), a short training run's held-out RMSE,
-and the options that path does not offer are refused."""
+FP32 Gram + FP32 Cholesky in LDS; rows of at most 95 ratings in the dual
+form, one wavefront per row), through the C ABI, against the FP64 oracle:
+factors within 1e-3 relative, per-row relative residual against the
+oracle's FP64 system <= 1e-4 (<= 1e-5 for the dual-form rows), the dual
+form against the primal kernel on the same rows, empty rows, the drop-in
+entry point (also with f % 4 != 0, the re-strided path), a short training
+run's held-out RMSE, and the options that path does not offer are
+refused."""
 import functools
 
 import numpy as np
@@ -49,6 +52,71 @@ def test_large_f_half_sweeps_match_oracle(f):
         assert h.stats()["failed_rows"] == 0
     _sweep_check(ds.csr_row_ptr, ds.csr_col, ds.csr_val, T0, X1, f, lam, [0, 1, 2, 399])
     _sweep_check(ds.csc_col_ptr, ds.csc_row, ds.csc_val, X1, T1, f, lam, [0, 1, 2])
+
+
+def _x_sweep(ds, f, lam, T0, dual):
+    prev = G.lib().cumf_debug_large_dual(int(dual))
+    try:
+        with G.ALS(f, lam) as h:
+            h.set_dataset(ds)
+            h.set_factors(theta=T0)
+            h.update_x()
+            X, _ = h.get_factors()
+            assert h.stats()["failed_rows"] == 0
+    finally:
+        G.lib().cumf_debug_large_dual(prev)
+    return X
+
+
+@pytest.mark.parametrize("f", [144, 256])
+def test_large_f_dual_form_matches_primal_and_oracle(f):
+    """Rows of at most 95 ratings: the dual form (K = Y Y^T on FP32 MFMA, a
+    Cholesky of at most six blocks, x = Y^T z) against the primal kernel
+    (switched on for every row) and the oracle's FP64 system; heavier rows
+    are the primal kernel's either way, bit for bit."""
+    ds = _mid()
+    lam = 0.058
+    T0 = cumf.init_factors(ds.n, f, 21)
+    Xp = _x_sweep(ds, f, lam, T0, dual=False)
+    Xd = _x_sweep(ds, f, lam, T0, dual=True)
+    deg = np.diff(ds.csr_row_ptr)
+    light, heavy = np.nonzero(deg <= 95)[0], np.nonzero(deg > 95)[0]
+    assert len(light) >= 100 and len(heavy) >= 10, (len(light), len(heavy))
+    np.testing.assert_array_equal(Xd[heavy], Xp[heavy])
+    rel = np.linalg.norm(Xd[light] - Xp[light], axis=1) / np.linalg.norm(Xp[light], axis=1)
+    assert rel.max() <= 1e-4, (rel.max(), int(light[rel.argmax()]))
+    picks = sorted(set(light[:: max(1, len(light) // 48)]) | {int(light[deg[light].argmin()]),
+                                                             int(light[deg[light].argmax()])})
+    for u in picks:
+        A, b = O.row_system(ds.csr_row_ptr, ds.csr_col, ds.csr_val, T0, f, lam, u)
+        res = np.linalg.norm(b - A @ Xd[u].astype(np.float64)) / np.linalg.norm(b)
+        assert res <= 1e-5, (u, int(deg[u]), res)
+
+
+def test_large_f_empty_and_light_rows():
+    """Empty rows give x = 0; one row per dual block count (1 .. 95
+    ratings) and a heavier one, against the oracle."""
+    f, lam, n = 160, 0.1, 200
+    rng = np.random.default_rng(4)
+    counts = [0, 1, 15, 16, 31, 0, 47, 63, 64, 79, 95, 96, 150]
+    rp = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(n, c, replace=False)) for c in counts]).astype(np.int32)
+    cv = rng.integers(1, 6, size=ci.shape[0]).astype(np.float32)
+    T0 = cumf.init_factors(n, f, 7)
+    m = len(counts)
+    with G.ALS(f, lam) as h:
+        h.set_ratings(m, n, rp, ci, cv)
+        h.set_factors(theta=T0)
+        h.update_x()
+        X, _ = h.get_factors()
+        assert h.stats()["failed_rows"] == 0
+    assert np.all(X[[0, 5]] == 0)
+    for u in range(m):
+        if counts[u] == 0:
+            continue
+        A, b = O.row_system(rp, ci, cv, T0, f, lam, u)
+        res = np.linalg.norm(b - A @ X[u].astype(np.float64)) / np.linalg.norm(b)
+        assert res <= 1e-5, (u, counts[u], res)
 
 
 @pytest.mark.parametrize("f", [150, 200])
