@@ -277,10 +277,11 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
     }
     __syncthreads();  // the panel is final
     // Lookahead: wavefront 0 updates the next diagonal tile and factors it
-    // while wavefronts 1-3 apply the rest of block K's trailing update
-    // (tiles (I, J), K < I <= J, dealt round-robin).  Only the last block
-    // can hold no pivot of the f x f system; its diagonal tile is then not
-    // factored (its x is zero).
+    // while wavefronts 1-3 apply the rest of block K's trailing update:
+    // each row I = K+1 .. NB-1 is cut into runs of four tiles (I, J),
+    // J >= I, dealt round-robin; a full run's four MFMA chains are
+    // interleaved.  Only the last block can hold no pivot of the f x f
+    // system; its diagonal tile is then not factored (its x is zero).
     const int K1 = K + 1;
     if (wv == 0) {
       trailing_tile(K, K1, K1);
@@ -291,9 +292,29 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
       }
     } else {
       int q = 0;
-      for (int I = K1; I < NB; ++I)
-        for (int J = (I == K1 ? K1 + 1 : I); J < NB; ++J, ++q)
-          if (q % 3 == wv - 1) trailing_tile(K, I, J);
+      for (int I = K1; I < NB; ++I) {
+        for (int J0 = I == K1 ? K1 + 1 : I; J0 < NB; J0 += 4, ++q) {
+          if (q % 3 != wv - 1) continue;
+          if (J0 + 4 > NB) {
+            for (int J = J0; J < NB; ++J) trailing_tile(K, I, J);
+            continue;
+          }
+          const f4 uv = *reinterpret_cast<const f4*>(slot(K, I) + 4 * lane);
+          const f4 nu = {-uv[0], -uv[1], -uv[2], -uv[3]};
+          f4 v[4], d[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            v[t] = *reinterpret_cast<const f4*>(slot(K, J0 + t) + 4 * lane);
+            d[t] = *reinterpret_cast<const f4*>(slot(I, J0 + t) + 4 * lane);
+          }
+#pragma unroll
+          for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(nu[s], v[t][s], d[t], 0, 0, 0);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) *reinterpret_cast<f4*>(slot(I, J0 + t) + 4 * lane) = d[t];
+        }
+      }
     }
     __syncthreads();  // block K's trailing update and W_{K+1} are in LDS
   }
