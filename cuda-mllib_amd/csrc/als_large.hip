@@ -245,35 +245,64 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
   __syncthreads();  // all tiles in their slots
 
   auto slot = [&](int I, int J) { return lds + tile_index(I, J, NB) * 256; };
-  // A_IJ -= U_KI^T U_KJ on one tile.
-  auto trailing_tile = [&](int K, int I, int J) {
+  // A_IJ -= U_KI^T U_KJ on the N tiles (I, J0) .. (I, J0+N-1), their MFMA
+  // chains interleaved.
+  auto trailing_run = [&](int K, int I, int J0, auto n) {
+    constexpr int N = decltype(n)::value;
     const f4 uv = *reinterpret_cast<const f4*>(slot(K, I) + 4 * lane);
-    const f4 v = *reinterpret_cast<const f4*>(slot(K, J) + 4 * lane);
-    float* p = slot(I, J);
-    f4 d = *reinterpret_cast<const f4*>(p + 4 * lane);
+    const f4 nu = {-uv[0], -uv[1], -uv[2], -uv[3]};
+    f4 v[N], d[N];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) d = __builtin_amdgcn_mfma_f32_16x16x4f32(-uv[s], v[s], d, 0, 0, 0);
-    *reinterpret_cast<f4*>(p + 4 * lane) = d;
+    for (int t = 0; t < N; ++t) {
+      v[t] = *reinterpret_cast<const f4*>(slot(K, J0 + t) + 4 * lane);
+      d[t] = *reinterpret_cast<const f4*>(slot(I, J0 + t) + 4 * lane);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int t = 0; t < N; ++t) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(nu[s], v[t][s], d[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < N; ++t) *reinterpret_cast<f4*>(slot(I, J0 + t) + 4 * lane) = d[t];
   };
+  // U_KJ = W_K A_KJ on the N tiles J0, J0+4, ..., likewise; A operand
+  // W_K[c][4g+s] from the column-major W.
+  auto panel_run = [&](int K, int J0, auto n) {
+    constexpr int N = decltype(n)::value;
+    const float* w = slot(K, K);
+    f4 wa;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) wa[s] = w[16 * (4 * g + s) + c];
+    f4 b[N], d[N];
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+      b[t] = *reinterpret_cast<const f4*>(slot(K, J0 + 4 * t) + 4 * lane);
+      d[t] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int t = 0; t < N; ++t) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[s], b[t][s], d[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < N; ++t) *reinterpret_cast<f4*>(slot(K, J0 + 4 * t) + 4 * lane) = d[t];
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
   // Block 0's diagonal tile (NB >= 10 here: block 0 is neither the last
   // nor short of pivots).
   if (wv == 0) diag_tile<NB>(slot(0, 0), slot(0, 0), tsc, ident, f, false, lane);
   __syncthreads();  // W_0 is in LDS
   for (int K = 0; K + 1 < NB; ++K) {
     {
-      // Panel: U_KJ = W_K A_KJ; A operand W_K[c][4g+s] from the column-major W.
-      const float* w = slot(K, K);
-      f4 wa;
-#pragma unroll
-      for (int s = 0; s < 4; ++s) wa[s] = w[16 * (4 * g + s) + c];
-      for (int J = K + 1 + wv; J < NB; J += 4) {
-        float* p = slot(K, J);
-        const f4 b = *reinterpret_cast<const f4*>(p + 4 * lane);
-        f4 d = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) d = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[s], b[s], d, 0, 0, 0);
-        *reinterpret_cast<f4*>(p + 4 * lane) = d;
-      }
+      // Panel: wavefront wv forms tiles J = K+1+wv, +4, ... (at most four:
+      // NB <= 17).
+      const int J0 = K + 1 + wv;
+      const int n = J0 < NB ? (NB - J0 + 3) / 4 : 0;
+      if (n == 4) panel_run(K, J0, I4{});
+      else if (n == 3) panel_run(K, J0, I3{});
+      else if (n == 2) panel_run(K, J0, I2{});
+      else if (n == 1) panel_run(K, J0, I1{});
     }
     __syncthreads();  // the panel is final
     // Lookahead: wavefront 0 updates the next diagonal tile and factors it
@@ -284,7 +313,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
     // system; its diagonal tile is then not factored (its x is zero).
     const int K1 = K + 1;
     if (wv == 0) {
-      trailing_tile(K, K1, K1);
+      trailing_run(K, K1, K1, I1{});
       lds_order();
       if (16 * K1 < f) {
         const bool last = K1 + 1 == NB;
@@ -295,24 +324,11 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
       for (int I = K1; I < NB; ++I) {
         for (int J0 = I == K1 ? K1 + 1 : I; J0 < NB; J0 += 4, ++q) {
           if (q % 3 != wv - 1) continue;
-          if (J0 + 4 > NB) {
-            for (int J = J0; J < NB; ++J) trailing_tile(K, I, J);
-            continue;
-          }
-          const f4 uv = *reinterpret_cast<const f4*>(slot(K, I) + 4 * lane);
-          const f4 nu = {-uv[0], -uv[1], -uv[2], -uv[3]};
-          f4 v[4], d[4];
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            v[t] = *reinterpret_cast<const f4*>(slot(K, J0 + t) + 4 * lane);
-            d[t] = *reinterpret_cast<const f4*>(slot(I, J0 + t) + 4 * lane);
-          }
-#pragma unroll
-          for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(nu[s], v[t][s], d[t], 0, 0, 0);
-#pragma unroll
-          for (int t = 0; t < 4; ++t) *reinterpret_cast<f4*>(slot(I, J0 + t) + 4 * lane) = d[t];
+          const int n = NB - J0;
+          if (n >= 4) trailing_run(K, I, J0, I4{});
+          else if (n == 3) trailing_run(K, I, J0, I3{});
+          else if (n == 2) trailing_run(K, I, J0, I2{});
+          else trailing_run(K, I, J0, I1{});
         }
       }
     }
