@@ -48,7 +48,8 @@ typedef struct ihipStream_t* hipStream_t;
  *     kernels here need no per-row f x f buffer, so it is only validated.
  * Differences (documented improvements):
  *   - any 1 <= f <= 256 (the reference needs f % 10 == 0, als/README.md:63);
- *     f > 140 runs a work-group-per-row FP32 Gram + Cholesky kernel;
+ *     f > 140 runs a work-group-per-row FP32 Gram + Cholesky kernel, and
+ *     the FP32 dual form for rows of at most 95 ratings;
  *   - rows with no ratings get x_u = 0 (the reference divides by zero);
  *   - errors never exit(): the call returns early and cumf_last_error()
  *     describes the failure (the CSR arrays are still released).
