@@ -92,7 +92,7 @@ def _dual_table(tmp_path, split, dual_variant=2):
     return p
 
 
-@pytest.mark.parametrize("f", [10, 32, 100])
+@pytest.mark.parametrize("f", [10, 32, 100, 128, 140])
 def test_dual_form_matches_primal_and_oracle(tmp_path, f):
     """Rows of at most 95 ratings (degree buckets 0-2: two staged chunks up
     to 63 ratings, three up to 95) solved in the dual form,
