@@ -213,7 +213,8 @@ template <int NB>
 __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void als_large_kernel(SweepArgs a) {
   extern __shared__ float lds[];
   constexpr int NT = num_tiles(NB);
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // (the wavefront index is uniform: branches on it are scalar)
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   // Whole rows: task blockIdx.x of a.tasks, or row blockIdx.x without a list.
   const int64_t u = a.tasks ? (int64_t)a.tasks[blockIdx.x].row : (int64_t)blockIdx.x;
@@ -320,16 +321,21 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
         diag_tile<NB>(slot(K1, K1), last ? wlast : slot(K1, K1), tsc, ident, f - 16 * K1, last, lane);
       }
     } else {
-      int q = 0;
+      // Runs are numbered row by row; this wavefront takes runs
+      // q = wv - 1 (mod 3).
+      int q0 = 0;  // number of the row's first run
       for (int I = K1; I < NB; ++I) {
-        for (int J0 = I == K1 ? K1 + 1 : I; J0 < NB; J0 += 4, ++q) {
-          if (q % 3 != wv - 1) continue;
+        const int j_first = I == K1 ? K1 + 1 : I;
+        const int nr = (NB - j_first + 3) / 4;
+        for (int r = ((wv - 1 - q0) % 3 + 3) % 3; r < nr; r += 3) {
+          const int J0 = j_first + 4 * r;
           const int n = NB - J0;
           if (n >= 4) trailing_run(K, I, J0, I4{});
           else if (n == 3) trailing_run(K, I, J0, I3{});
           else if (n == 2) trailing_run(K, I, J0, I2{});
           else trailing_run(K, I, J0, I1{});
         }
+        q0 += nr;
       }
     }
     __syncthreads();  // block K's trailing update and W_{K+1} are in LDS
