@@ -108,6 +108,9 @@ int plan_tasks(const int32_t* row_ptr, int64_t m, int split_, int f, std::vector
 struct Plan {
   int64_t rows = 0, nnz = 0, ntasks = 0, nheavy = 0, nslots = 0;
   int64_t light0 = 0;  // first task of at most kDualMaxRatingsHost ratings (tasks light0 .. ntasks-1)
+  // Tasks [wg_t0[d], wg_t1[d]) have more than kDualMaxRatingsHost ratings
+  // and d = (count + 16) / 16 <= kLargeDualWgMaxBlocks row blocks.
+  int64_t wg_t0[cumf::kLargeDualWgMaxBlocks + 1] = {}, wg_t1[cumf::kLargeDualWgMaxBlocks + 1] = {};
   int split = 0;
   int64_t bucket_t0[kBuckets] = {}, bucket_t1[kBuckets] = {};
   DevBuf tasks, heavy, partial;
@@ -131,6 +134,14 @@ struct Plan {
     light0 = ntasks;
     for (int b = 0; b < kBuckets && kBucketEdge[b + 1] <= cumf::kDualMaxRatingsHost + 1; ++b)
       if (bucket_t1[b] > bucket_t0[b]) light0 = std::min(light0, bucket_t0[b]);
+    for (int d = 0; d <= cumf::kLargeDualWgMaxBlocks; ++d) wg_t0[d] = wg_t1[d] = 0;
+    for (int64_t i = 0; i < ntasks; ++i) {
+      const int cnt = t[(size_t)i].count;
+      const int d = (cnt + 16) / 16;
+      if (cnt <= cumf::kDualMaxRatingsHost || d > cumf::kLargeDualWgMaxBlocks) continue;
+      if (wg_t1[d] == 0) wg_t0[d] = i;
+      wg_t1[d] = i + 1;
+    }
     nheavy = (int64_t)hv.size();
     nslots = slots;
     CUMF_TRY(tasks.alloc(sizeof(cumf::Task) * t.size()));
@@ -351,29 +362,62 @@ hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::Sw
   return task_launch(x, cfg.variant, stream);
 }
 
-// The half-sweep above kMaxFusedF over a plan's whole-row tasks: tasks
-// 0 .. nh-1 on the work-group-per-row kernel, the nl tasks after them (at
-// most kDualMaxRatingsHost ratings each) in the dual form, side by side
-// when an auxiliary stream is given (forked from s and joined back).
-hipError_t large_launches(const cumf::SweepArgs& a, int64_t nh, int64_t nl, hipStream_t s, hipStream_t aux = nullptr,
-                          hipEvent_t fork = nullptr, hipEvent_t join = nullptr) {
-  cumf::SweepArgs d = a;
-  d.tasks = a.tasks + nh;
-  const bool both = aux && nh > 0 && nl > 0;
+// The half-sweep above kMaxFusedF over a plan's whole-row tasks (heaviest
+// first), with the dual forms on (g_large_dual):
+//   * rows of at most kDualMaxRatingsHost ratings: the wavefront dual form;
+//   * rows whose dual system has at most NB - 2 row blocks (NB those of the
+//     f x f system, and at most kLargeDualWgMaxBlocks): the work-group dual
+//     form, one launch per block count;
+//   * the rest: the work-group primal kernel.
+// Side by side when auxiliary streams are given (forked from s, joined
+// back); launches counts the kernel launches.
+hipError_t large_launches(const cumf::SweepArgs& a, const Plan& p, int* launches, hipStream_t s,
+                          hipStream_t aux0 = nullptr, hipStream_t aux1 = nullptr, hipEvent_t fork = nullptr,
+                          hipEvent_t join0 = nullptr, hipEvent_t join1 = nullptr) {
+  const bool dual = g_large_dual.load() != 0;
+  const int nb = cumf::sweep_nb(a.f);
+  int64_t primal_end = dual ? p.light0 : p.ntasks;  // tasks [0, primal_end) take the primal kernel
+  int dmax = 0;                                     // the work-group dual form's largest block count
+  if (dual)
+    for (int d = std::min(nb - 2, cumf::kLargeDualWgMaxBlocks); d >= 1; --d)
+      if (p.wg_t1[d] > p.wg_t0[d]) {
+        dmax = std::max(dmax, d);
+        primal_end = std::min(primal_end, p.wg_t0[d]);
+      }
+  const int64_t nl = dual ? p.ntasks - p.light0 : 0;
+  const bool side = aux0 && aux1;
   hipError_t e = hipSuccess;
-  if (both) {
+  if (side) {
     e = hipEventRecord(fork, s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
+    if (e == hipSuccess) e = hipStreamWaitEvent(aux0, fork, 0);
+    if (e == hipSuccess) e = hipStreamWaitEvent(aux1, fork, 0);
   }
-  if (e == hipSuccess && nh > 0) e = cumf::launch_large_kernel(a, nh, s);
-  if (e == hipSuccess && nl > 0) e = cumf::launch_large_dual_kernel(d, nl, both ? aux : s);
-  if (e == hipSuccess && both) {
-    e = hipEventRecord(join, aux);
-    if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
+  *launches = 0;
+  if (e == hipSuccess && primal_end > 0) {
+    e = cumf::launch_large_kernel(a, primal_end, s);
+    ++*launches;
+  }
+  for (int d = dmax; e == hipSuccess && d >= 1; --d) {
+    if (p.wg_t1[d] <= p.wg_t0[d]) continue;
+    cumf::SweepArgs w = a;
+    w.tasks = a.tasks + p.wg_t0[d];
+    e = cumf::launch_large_dual_wg_kernel(w, p.wg_t1[d] - p.wg_t0[d], d, side ? aux1 : s);
+    ++*launches;
+  }
+  if (e == hipSuccess && nl > 0) {
+    cumf::SweepArgs l = a;
+    l.tasks = a.tasks + p.light0;
+    e = cumf::launch_large_dual_kernel(l, nl, side ? aux0 : s);
+    ++*launches;
+  }
+  if (e == hipSuccess && side) {
+    e = hipEventRecord(join0, aux0);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, join0, 0);
+    if (e == hipSuccess) e = hipEventRecord(join1, aux1);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, join1, 0);
   }
   return e;
 }
-int64_t large_dual_tasks(const Plan& p) { return g_large_dual.load() ? p.ntasks - p.light0 : 0; }
 
 // All tasks of a half-sweep: one launch, or with a tile table one launch
 // per non-empty degree bucket (heaviest first), persistent ones after their
@@ -543,16 +587,19 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
   cumf::SweepArgs a{};
   CUMF_TRY(fill_args(h, p, rowp, col, val, theta, x, theta_b, x_b, trace, a));
   if (h->f > kMaxFusedF) {
-    const int64_t nl = large_dual_tasks(p), nh = p.ntasks - nl;
     hipEvent_t s0 = nullptr, s1 = nullptr;
     if (h->opt.timing && !h->capturing) {
       s0 = h->timer.get();
       s1 = h->timer.get();
       if (s0) (void)hipEventRecord(s0, h->stream);
     }
-    CUMF_TRY(timed(h, Timer::TASK,
-                   [&] { return large_launches(a, nh, nl, h->stream, h->aux[0], h->fork_ev, h->join_ev[0]); },
-                   std::max(1, (nh > 0) + (nl > 0))));
+    int nlaunch = 0;
+    CUMF_TRY(timed(h, Timer::TASK, [&] {
+      return large_launches(a, p, &nlaunch, h->stream, h->aux[0], h->aux[1], h->fork_ev, h->join_ev[0],
+                            h->join_ev[1]);
+    }));
+    if (h->opt.timing && !h->capturing && !h->timer.pending.empty())
+      h->timer.pending.back().launches = std::max(1, nlaunch);
     if (s0 && s1) {
       (void)hipEventRecord(s1, h->stream);
       h->timer.pending.push_back({s0, s1, Timer::SWEEP, 1});
@@ -1877,8 +1924,8 @@ int half_sweep_device(hipStream_t stream, const int32_t* d_row_ptr, const int32_
     CUMF_HIP_TRY(cumf::launch_compat(c, m, stream));
   } else if (f > kMaxFusedF) {
     CUMF_HIP_TRY(cumf::prepare_large_kernel(f));
-    const int64_t nl = large_dual_tasks(*plan);
-    CUMF_HIP_TRY(large_launches(a, plan->ntasks - nl, nl, stream));
+    int nlaunch = 0;
+    CUMF_HIP_TRY(large_launches(a, *plan, &nlaunch, stream));
   } else {
     CUMF_HIP_TRY(task_launch(a, default_variant(f), stream));
     CUMF_HIP_TRY(finish_launch(a, default_variant(f), stream));

@@ -139,6 +139,12 @@ size_t large_lds_bytes(int f);
 // The dual form for 140 < f <= 256: one wavefront per whole-row task
 // a.tasks[0 .. ntasks), each of at most kDualMaxRatingsHost ratings (FP32).
 hipError_t launch_large_dual_kernel(const SweepArgs& a, int64_t ntasks, hipStream_t s);
+// The dual form on the work-group for 140 < f <= 256: whole-row tasks
+// a.tasks[0 .. ntasks) of more than kDualMaxRatingsHost ratings whose
+// systems have nbd = (n + 16) / 16 row blocks (7 .. kLargeDualWgMaxBlocks).
+// prepare_large_kernel raises the LDS limits of these kernels too.
+constexpr int kLargeDualWgMaxBlocks = 16;
+hipError_t launch_large_dual_wg_kernel(const SweepArgs& a, int64_t ntasks, int nbd, hipStream_t s);
 hipError_t launch_dump_kernel(const SweepArgs& a, const int32_t* rows, int64_t nrows, float* out,
                               hipStream_t s);
 hipError_t launch_rmse(const int32_t* u, const int32_t* v, const float* r, int64_t cnt, const float* X,

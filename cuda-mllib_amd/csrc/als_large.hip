@@ -55,6 +55,10 @@ namespace {
 
 constexpr int kLT = 256;  // threads per row: four wavefronts
 constexpr int kLargeMaxF = 256;
+// Row blocks of the work-group dual form: rows of 96 .. 255 ratings.
+constexpr int kLargeDualMinBlocks = (kDualMaxRatingsHost + 1 + 16) / 16;
+constexpr int kLargeDualMaxBlocks = kLargeDualWgMaxBlocks;
+static_assert(kLargeDualMinBlocks == 7 && kLargeDualMaxBlocks == 16, "blocks 7 .. 16");
 
 __host__ __device__ constexpr int qmax(int NB) { return (num_tiles(NB) + 3) / 4; }
 
@@ -71,6 +75,8 @@ __device__ __forceinline__ void static_for(F&& fn) {
 // last block's W, z and x.  The Gram's staging (2 chunks of NB tiles and
 // their ratings) overlays the tile area.
 __host__ __device__ constexpr int large_lds_floats(int NB) { return num_tiles(NB) * 256 + 3 * 256 + 32 * NB; }
+// The dual form also keeps the row's ratings and column indices (256 each).
+__host__ __device__ constexpr int large_dual_wg_lds_floats(int NB) { return large_lds_floats(NB) + 512; }
 
 // Tile t of the upper triangle in tile_index order -> (I, J).
 __host__ __device__ constexpr int tile_row(int t, int NB) {
@@ -176,6 +182,77 @@ __device__ __forceinline__ void gram_tiles(const SweepArgs& a, int64_t start, in
   });
 }
 
+// Dual form on the work-group (rows of n <= 16 NB - 1 ratings):
+// wavefront W's tiles of K = Y Y^T (t = W, W + 4, ...), over 16-column
+// slabs of the factor rows listed in cs (lane (g, c) loads columns
+// 4g .. 4g+3 of row 16I + c of every row block I, the next slab's loads
+// issued before the current slab's MFMAs); then r into column 16 NB - 1,
+// mu = lambda n on the first n diagonal entries, and the store of the
+// tiles to their LDS slots.
+template <int NB, int W>
+__device__ __forceinline__ void dual_tiles(const SweepArgs& a, int n, const int* cs, const float* rs, float* lds,
+                                           int lane) {
+  constexpr int NT = num_tiles(NB), Q = qmax(NB);
+  const int g = lane >> 4, c = lane & 15;
+  const int ldf = (int)a.ldf;
+  const int S = (ldf + 15) >> 4;  // 16-column slabs; columns past ldf are zero
+  const float* rp[NB];
+#pragma unroll
+  for (int I = 0; I < NB; ++I) {
+    const int r = 16 * I + c;
+    rp[I] = r < n ? a.theta + (int64_t)cs[r] * ldf : kZeroRow;  // (kZeroRow holds 256 >= ldf floats)
+  }
+  auto load = [&](int sl, f4 (&v)[NB]) {
+    const int k = 16 * sl + 4 * g;
+#pragma unroll
+    for (int I = 0; I < NB; ++I)
+      v[I] = k < ldf ? *reinterpret_cast<const f4*>(rp[I] + k) : f4{0.f, 0.f, 0.f, 0.f};
+  };
+  f4 acc[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) acc[q] = f4{0.f, 0.f, 0.f, 0.f};
+  auto mfmas = [&](const f4 (&v)[NB]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      static_for<Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value, t = 4 * q + W;
+        if constexpr (t < NT) {
+          constexpr int I = tile_row(t, NB), J = tile_col(t, NB);
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[I][j], v[J][j], acc[q], 0, 0, 0);
+        }
+      });
+  };
+  f4 va[NB], vb[NB];
+  load(0, va);
+  for (int sl = 0; sl < S; sl += 2) {
+    if (sl + 1 < S) load(sl + 1, vb);
+    mfmas(va);
+    if (sl + 1 < S) {
+      if (sl + 2 < S) load(sl + 2, va);
+      mfmas(vb);
+    }
+  }
+  const float ridge = a.lambda * (float)n;
+  static_for<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value, t = 4 * q + W;
+    if constexpr (t < NT) {
+      constexpr int I = tile_row(t, NB), J = tile_col(t, NB);
+      if constexpr (J == NB - 1) {
+        // r into column 16 NB - 1 (K is zero there: rows past n are zero)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (c == 15) acc[q][r] = rs[16 * I + 4 * g + r];
+      }
+      if constexpr (I == J) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (c == 4 * g + r && 16 * I + c < n) acc[q][r] += ridge;
+      }
+      *reinterpret_cast<f4*>(lds + t * 256 + 4 * lane) = acc[q];
+    }
+  });
+}
+
 // Diagonal tile K (in accumulator order in its slot) -> W_K = U_KK^{-T},
 // column-major, into wdst; for the last block also U_KK back into the slot
 // (its augmented columns hold z).  Wavefront 0 only.
@@ -209,7 +286,10 @@ __device__ __forceinline__ void diag_tile(float* slot, float* wdst, float* tsc, 
   if (last) *reinterpret_cast<f4*>(slot + 4 * lane) = *reinterpret_cast<const f4*>(tsc + 16 * c + 4 * g);
 }
 
-template <int NB>
+// DUAL: the tasks are rows of n ratings, 16 (NB - 1) <= n <= 16 NB - 1,
+// solved in the dual form on NB row blocks (K = Y Y^T with r in column
+// 16 NB - 1, then x = Y^T z).
+template <int NB, bool DUAL>
 __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void als_large_kernel(SweepArgs a) {
   extern __shared__ float lds[];
   constexpr int NT = num_tiles(NB);
@@ -222,26 +302,49 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
   const int64_t s0 = a.tasks ? (int64_t)a.tasks[blockIdx.x].start : (int64_t)a.row_ptr[u];
   const int cnt = a.tasks ? a.tasks[blockIdx.x].count : a.row_ptr[u + 1] - a.row_ptr[u];
   float* xrow = a.x + u * a.ldf;
-  if (cnt <= 0) {
+  if (cnt <= 0 || (DUAL && cnt > 16 * NB - 1)) {  // (the host launches only rows that fit: n + 1 <= 16 NB)
     for (int i = threadIdx.x; i < (int)a.ldf; i += kLT) xrow[i] = 0.0f;
+    if (cnt > 0 && threadIdx.x == 0) atomicAdd(a.fail_counter, 1);
     return;
   }
+  // Size of the system: f, or n in the dual form; its augmented column.
+  const int fs = DUAL ? cnt : f;
+  const int cb = DUAL ? 15 : (int)a.ldf - 16 * (NB - 1);
   float* tsc = lds + NT * 256;
   float* ident = tsc + 256;
   float* wlast = ident + 256;
   float* zb = wlast + 256;
   float* xs = zb + 16 * NB;
+  float* rs = xs + 16 * NB;  // dual form: the ratings and their column indices
+  int* cs = reinterpret_cast<int*>(rs + 256);
+  if constexpr (DUAL) {
+    for (int i = threadIdx.x; i < 256; i += kLT) {
+      const bool in = i < cnt;
+      rs[i] = in ? a.val[s0 + i] : 0.0f;
+      cs[i] = in ? a.col[s0 + i] : 0;
+    }
+    __syncthreads();
+  }
   if (wv == 0) {
     const f4 e = {4 * g == c ? 1.0f : 0.0f, 4 * g + 1 == c ? 1.0f : 0.0f, 4 * g + 2 == c ? 1.0f : 0.0f,
                   4 * g + 3 == c ? 1.0f : 0.0f};
     *reinterpret_cast<f4*>(ident + 16 * c + 4 * g) = e;
   }
 
-  switch (wv) {
-    case 0: gram_tiles<NB, 0>(a, s0, cnt, lds, lane); break;
-    case 1: gram_tiles<NB, 1>(a, s0, cnt, lds, lane); break;
-    case 2: gram_tiles<NB, 2>(a, s0, cnt, lds, lane); break;
-    default: gram_tiles<NB, 3>(a, s0, cnt, lds, lane); break;
+  if constexpr (DUAL) {
+    switch (wv) {
+      case 0: dual_tiles<NB, 0>(a, cnt, cs, rs, lds, lane); break;
+      case 1: dual_tiles<NB, 1>(a, cnt, cs, rs, lds, lane); break;
+      case 2: dual_tiles<NB, 2>(a, cnt, cs, rs, lds, lane); break;
+      default: dual_tiles<NB, 3>(a, cnt, cs, rs, lds, lane); break;
+    }
+  } else {
+    switch (wv) {
+      case 0: gram_tiles<NB, 0>(a, s0, cnt, lds, lane); break;
+      case 1: gram_tiles<NB, 1>(a, s0, cnt, lds, lane); break;
+      case 2: gram_tiles<NB, 2>(a, s0, cnt, lds, lane); break;
+      default: gram_tiles<NB, 3>(a, s0, cnt, lds, lane); break;
+    }
   }
   __syncthreads();  // all tiles in their slots
 
@@ -290,9 +393,9 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
   using I2 = std::integral_constant<int, 2>;
   using I3 = std::integral_constant<int, 3>;
   using I4 = std::integral_constant<int, 4>;
-  // Block 0's diagonal tile (NB >= 10 here: block 0 is neither the last
-  // nor short of pivots).
-  if (wv == 0) diag_tile<NB>(slot(0, 0), slot(0, 0), tsc, ident, f, false, lane);
+  // Block 0's diagonal tile (NB >= 7 here, and the system has at least 96
+  // pivots: block 0 is neither the last nor short of pivots).
+  if (wv == 0) diag_tile<NB>(slot(0, 0), slot(0, 0), tsc, ident, fs, false, lane);
   __syncthreads();  // W_0 is in LDS
   for (int K = 0; K + 1 < NB; ++K) {
     {
@@ -316,9 +419,9 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
     if (wv == 0) {
       trailing_run(K, K1, K1, I1{});
       lds_order();
-      if (16 * K1 < f) {
+      if (16 * K1 < fs) {
         const bool last = K1 + 1 == NB;
-        diag_tile<NB>(slot(K1, K1), last ? wlast : slot(K1, K1), tsc, ident, f - 16 * K1, last, lane);
+        diag_tile<NB>(slot(K1, K1), last ? wlast : slot(K1, K1), tsc, ident, fs - 16 * K1, last, lane);
       }
     } else {
       // Runs are numbered row by row; this wavefront takes runs
@@ -342,8 +445,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
   }
   if (wv != 0) return;
 
-  // z out of the augmented column (column ldf of the system, in block NB-1).
-  const int cb = (int)a.ldf - 16 * (NB - 1);
+  // z out of the augmented column (column cb of block NB-1).
   float* sink = tsc + 4 * lane;
   for (int I = 0; I < NB; ++I) {
     const f4 v = *reinterpret_cast<const f4*>(slot(I, NB - 1) + 4 * lane);
@@ -369,7 +471,7 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
     float x = 0.0f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) x = fmaf(wq[j >> 2][j & 3], readlane_f(t, 4 * j), x);
-    x = 16 * K + c < f ? x : 0.0f;  // a block past f has no W_K and gives x_K = 0
+    x = 16 * K + c < fs ? x : 0.0f;  // a block past the system's size has no W_K and gives x_K = 0
     fin = fin && fabsf(x) < __builtin_huge_valf();
     if (g == 0) xs[16 * K + c] = x;
     lds_order();
@@ -377,7 +479,38 @@ __global__ __launch_bounds__(kLT) __attribute__((amdgpu_waves_per_eu(1))) void a
   // A pivot that was not positive and finite leaves NaN or infinities in x.
   const bool ok = !__any(!fin);
   if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
-  for (int i = lane; i < (int)a.ldf; i += 64) xrow[i] = (ok && i < f) ? xs[i] : 0.0f;
+  if constexpr (DUAL) {
+    // x = Y^T z, z in xs: lane l forms columns 4l .. 4l+3.
+    const int ldf = (int)a.ldf;
+    const int k = 4 * lane;
+    if (k >= ldf) return;
+    f4 xv = {0.f, 0.f, 0.f, 0.f};
+    if (ok) {
+      int i = 0;
+      for (; i + 4 <= cnt; i += 4) {
+        f4 y[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y[q] = *reinterpret_cast<const f4*>(a.theta + (int64_t)cs[i + q] * ldf + k);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float zq = xs[i + q];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) xv[e] = fmaf(zq, y[q][e], xv[e]);
+        }
+      }
+      for (; i < cnt; ++i) {
+        const f4 y = *reinterpret_cast<const f4*>(a.theta + (int64_t)cs[i] * ldf + k);
+        const float zq = xs[i];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xv[e] = fmaf(zq, y[e], xv[e]);
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) xv[e] = k + e < f ? xv[e] : 0.0f;
+    *reinterpret_cast<f4*>(xrow + k) = xv;
+  } else {
+    for (int i = lane; i < (int)a.ldf; i += 64) xrow[i] = (ok && i < f) ? xs[i] : 0.0f;
+  }
 }
 
 // Dual form: row blocks of the largest system (95 ratings plus r's column),
@@ -503,11 +636,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void al
   if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
 }
 
-template <int NB>
+template <int NB, bool DUAL>
 hipError_t set_lds_limit() {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&als_large_kernel<NB>),
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&als_large_kernel<NB, DUAL>),
                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)(sizeof(float) * large_lds_floats(NB)));
+                             (int)(sizeof(float) * (DUAL ? large_dual_wg_lds_floats(NB) : large_lds_floats(NB))));
+}
+
+template <int NB, bool DUAL>
+void launch_nb(const SweepArgs& a, int64_t rows, hipStream_t s) {
+  hipLaunchKernelGGL((als_large_kernel<NB, DUAL>), dim3((unsigned)rows), dim3(kLT),
+                     sizeof(float) * (DUAL ? large_dual_wg_lds_floats(NB) : large_lds_floats(NB)), s, a);
 }
 
 }  // namespace
@@ -519,15 +658,21 @@ size_t large_lds_bytes(int f) {
 
 hipError_t prepare_large_kernel(int f) {
   if (f <= 0 || f > kLargeMaxF) return hipErrorInvalidValue;
+  // the work-group dual form, every block count it takes
+  for (hipError_t e : {set_lds_limit<kLargeDualMinBlocks, true>(), set_lds_limit<8, true>(),
+                       set_lds_limit<9, true>(), set_lds_limit<10, true>(), set_lds_limit<11, true>(),
+                       set_lds_limit<12, true>(), set_lds_limit<13, true>(), set_lds_limit<14, true>(),
+                       set_lds_limit<15, true>(), set_lds_limit<kLargeDualMaxBlocks, true>()})
+    if (e != hipSuccess) return e;
   switch (sweep_nb(f)) {
-    case 10: return set_lds_limit<10>();
-    case 11: return set_lds_limit<11>();
-    case 12: return set_lds_limit<12>();
-    case 13: return set_lds_limit<13>();
-    case 14: return set_lds_limit<14>();
-    case 15: return set_lds_limit<15>();
-    case 16: return set_lds_limit<16>();
-    case 17: return set_lds_limit<17>();
+    case 10: return set_lds_limit<10, false>();
+    case 11: return set_lds_limit<11, false>();
+    case 12: return set_lds_limit<12, false>();
+    case 13: return set_lds_limit<13, false>();
+    case 14: return set_lds_limit<14, false>();
+    case 15: return set_lds_limit<15, false>();
+    case 16: return set_lds_limit<16, false>();
+    case 17: return set_lds_limit<17, false>();
     default: return hipErrorInvalidValue;
   }
 }
@@ -541,20 +686,38 @@ hipError_t launch_large_dual_kernel(const SweepArgs& a, int64_t ntasks, hipStrea
   return hipGetLastError();
 }
 
+hipError_t launch_large_dual_wg_kernel(const SweepArgs& a, int64_t ntasks, int nbd, hipStream_t s) {
+  if (a.f <= 0 || a.f > kLargeMaxF || a.ldf > kLargeMaxF || ntasks < 0 || (ntasks > 0 && !a.tasks))
+    return hipErrorInvalidValue;
+  if (ntasks == 0) return hipSuccess;
+  switch (nbd) {
+    case 7: launch_nb<7, true>(a, ntasks, s); break;
+    case 8: launch_nb<8, true>(a, ntasks, s); break;
+    case 9: launch_nb<9, true>(a, ntasks, s); break;
+    case 10: launch_nb<10, true>(a, ntasks, s); break;
+    case 11: launch_nb<11, true>(a, ntasks, s); break;
+    case 12: launch_nb<12, true>(a, ntasks, s); break;
+    case 13: launch_nb<13, true>(a, ntasks, s); break;
+    case 14: launch_nb<14, true>(a, ntasks, s); break;
+    case 15: launch_nb<15, true>(a, ntasks, s); break;
+    case 16: launch_nb<16, true>(a, ntasks, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_large_kernel(const SweepArgs& a, int64_t rows, hipStream_t s) {
   if (a.f <= 0 || a.f > kLargeMaxF || rows < 0) return hipErrorInvalidValue;
   if (rows == 0) return hipSuccess;
-  const dim3 grid((unsigned)rows), block(kLT);
-  const size_t shmem = large_lds_bytes(a.f);
   switch (sweep_nb(a.f)) {
-    case 10: hipLaunchKernelGGL(als_large_kernel<10>, grid, block, shmem, s, a); break;
-    case 11: hipLaunchKernelGGL(als_large_kernel<11>, grid, block, shmem, s, a); break;
-    case 12: hipLaunchKernelGGL(als_large_kernel<12>, grid, block, shmem, s, a); break;
-    case 13: hipLaunchKernelGGL(als_large_kernel<13>, grid, block, shmem, s, a); break;
-    case 14: hipLaunchKernelGGL(als_large_kernel<14>, grid, block, shmem, s, a); break;
-    case 15: hipLaunchKernelGGL(als_large_kernel<15>, grid, block, shmem, s, a); break;
-    case 16: hipLaunchKernelGGL(als_large_kernel<16>, grid, block, shmem, s, a); break;
-    case 17: hipLaunchKernelGGL(als_large_kernel<17>, grid, block, shmem, s, a); break;
+    case 10: launch_nb<10, false>(a, rows, s); break;
+    case 11: launch_nb<11, false>(a, rows, s); break;
+    case 12: launch_nb<12, false>(a, rows, s); break;
+    case 13: launch_nb<13, false>(a, rows, s); break;
+    case 14: launch_nb<14, false>(a, rows, s); break;
+    case 15: launch_nb<15, false>(a, rows, s); break;
+    case 16: launch_nb<16, false>(a, rows, s); break;
+    case 17: launch_nb<17, false>(a, rows, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

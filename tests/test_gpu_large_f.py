@@ -1,6 +1,7 @@
 """GPU checks of the half-sweep for 140 < f <= 256 (one work-group per row,
-FP32 Gram + FP32 Cholesky in LDS; rows of at most 95 ratings in the dual
-form, one wavefront per row), through the C ABI, against the FP64 oracle:
+FP32 Gram + FP32 Cholesky in LDS; the dual form for lighter rows: up to 95
+ratings on one wavefront, above on the work-group while the dual system
+is the smaller), through the C ABI, against the FP64 oracle:
 factors within 1e-3 relative, per-row relative residual against the
 oracle's FP64 system <= 1e-4 (<= 1e-5 for the dual-form rows), the dual
 form against the primal kernel on the same rows, empty rows, the drop-in
@@ -68,37 +69,49 @@ def _x_sweep(ds, f, lam, T0, dual):
     return X
 
 
+def _sweep_nb(f):
+    ldf, ldb = (f + 3) & ~3, (f + 7) & ~7
+    return (max(ldf + 1, ldb, f + 2) + 15) // 16
+
+
 @pytest.mark.parametrize("f", [144, 256])
 def test_large_f_dual_form_matches_primal_and_oracle(f):
-    """Rows of at most 95 ratings: the dual form (K = Y Y^T on FP32 MFMA, a
-    Cholesky of at most six blocks, x = Y^T z) against the primal kernel
-    (switched on for every row) and the oracle's FP64 system; heavier rows
-    are the primal kernel's either way, bit for bit."""
+    """The dual forms against the primal kernel (switched on for every row)
+    and the oracle's FP64 system: rows of at most 95 ratings on one
+    wavefront (K = Y Y^T on FP32 MFMA, at most six blocks), rows whose dual
+    system has at most NB - 2 blocks on the work-group; heavier rows are
+    the primal kernel's either way, bit for bit."""
     ds = _mid()
     lam = 0.058
     T0 = cumf.init_factors(ds.n, f, 21)
     Xp = _x_sweep(ds, f, lam, T0, dual=False)
     Xd = _x_sweep(ds, f, lam, T0, dual=True)
     deg = np.diff(ds.csr_row_ptr)
-    light, heavy = np.nonzero(deg <= 95)[0], np.nonzero(deg > 95)[0]
-    assert len(light) >= 100 and len(heavy) >= 10, (len(light), len(heavy))
+    thr = 16 * min(_sweep_nb(f) - 2, 16) - 1  # longest row of the work-group dual form
+    light = np.nonzero(deg <= 95)[0]
+    medium = np.nonzero((deg > 95) & (deg <= thr))[0]
+    heavy = np.nonzero(deg > thr)[0]
+    assert len(light) >= 100 and len(medium) >= 10 and len(heavy) >= 5, (len(light), len(medium), len(heavy))
     np.testing.assert_array_equal(Xd[heavy], Xp[heavy])
-    rel = np.linalg.norm(Xd[light] - Xp[light], axis=1) / np.linalg.norm(Xp[light], axis=1)
-    assert rel.max() <= 1e-4, (rel.max(), int(light[rel.argmax()]))
-    picks = sorted(set(light[:: max(1, len(light) // 48)]) | {int(light[deg[light].argmin()]),
-                                                             int(light[deg[light].argmax()])})
-    for u in picks:
+    for rows in (light, medium):
+        rel = np.linalg.norm(Xd[rows] - Xp[rows], axis=1) / np.linalg.norm(Xp[rows], axis=1)
+        assert rel.max() <= 1e-4, (rel.max(), int(rows[rel.argmax()]))
+    picks = set(light[:: max(1, len(light) // 32)]) | set(medium[:: max(1, len(medium) // 24)])
+    for rows in (light, medium):
+        picks |= {int(rows[deg[rows].argmin()]), int(rows[deg[rows].argmax()])}
+    for u in sorted(picks):
         A, b = O.row_system(ds.csr_row_ptr, ds.csr_col, ds.csr_val, T0, f, lam, u)
         res = np.linalg.norm(b - A @ Xd[u].astype(np.float64)) / np.linalg.norm(b)
         assert res <= 1e-5, (u, int(deg[u]), res)
 
 
 def test_large_f_empty_and_light_rows():
-    """Empty rows give x = 0; one row per dual block count (1 .. 95
-    ratings) and a heavier one, against the oracle."""
+    """Empty rows give x = 0; rows at the block-count edges of both dual
+    forms (1 .. 95 ratings on a wavefront, 96 .. 143 on the work-group at
+    f = 160) and a heavier one, against the oracle."""
     f, lam, n = 160, 0.1, 200
     rng = np.random.default_rng(4)
-    counts = [0, 1, 15, 16, 31, 0, 47, 63, 64, 79, 95, 96, 150]
+    counts = [0, 1, 15, 16, 31, 0, 47, 63, 64, 79, 95, 96, 111, 112, 127, 143, 150]
     rp = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
     ci = np.concatenate([np.sort(rng.choice(n, c, replace=False)) for c in counts]).astype(np.int32)
     cv = rng.integers(1, 6, size=ci.shape[0]).astype(np.float32)
