@@ -49,7 +49,7 @@ typedef struct ihipStream_t* hipStream_t;
  * Differences (documented improvements):
  *   - any 1 <= f <= 256 (the reference needs f % 10 == 0, als/README.md:63);
  *     f > 140 runs a work-group-per-row FP32 Gram + Cholesky kernel, and
- *     the FP32 dual form for rows of at most 95 ratings;
+ *     the FP32 dual form for lighter rows;
  *   - rows with no ratings get x_u = 0 (the reference divides by zero);
  *   - errors never exit(): the call returns early and cumf_last_error()
  *     describes the failure (the CSR arrays are still released).
@@ -313,8 +313,10 @@ int cumf_debug_compat_system(cumf_handle* h, int side, const int32_t* rows, int6
  * out: host, rows x 4. */
 int cumf_debug_cg_trace(cumf_handle* h, int side, float* out);
 /* 0: above f = 140, every row takes the work-group-per-row primal kernel;
- * 1 (the default): rows of at most 95 ratings take the dual form.  Process
- * wide; returns the previous setting. */
+ * 1 (the default): rows of at most 95 ratings take the dual form on one
+ * wavefront, and rows whose dual system has at most two 16-column blocks
+ * fewer than the f x f one take it on a work-group.  Process wide; returns
+ * the previous setting. */
 int cumf_debug_large_dual(int on);
 /* 1: mallocBest skips device memory and takes its mapped-host fallback
  * (also set by the environment variable CUMF_FORCE_HOST_ALLOC=1). */
