@@ -70,9 +70,11 @@ struct BucketCfg {
 };
 
 // Host half of the planner: every row with at most `split` ratings is one
-// task; a longer row is cut into split-sized chunks, each a task writing
-// its partial Gram to a scratch slot (consecutive slots per row), and the
-// row is listed as heavy.  Above kMaxFusedF the work-group-per-row kernel
+// task; a longer row is cut into ceil(n / split) chunks whose sizes differ
+// by at most one rating (so none is short while split >= 2 x 96: no chunk
+// lands in the dual form's degree buckets), each a task writing its
+// partial Gram to a scratch slot (consecutive slots per row), and the row
+// is listed as heavy.  Above kMaxFusedF the work-group-per-row kernel
 // solves whole rows: nothing is split.  Tasks are ordered by decreasing
 // rating count (stable), so every degree bucket is a contiguous range.
 int plan_tasks(const int32_t* row_ptr, int64_t m, int split_, int f, std::vector<cumf::Task>& t,
@@ -92,10 +94,14 @@ int plan_tasks(const int32_t* row_ptr, int64_t m, int split_, int f, std::vector
       t.push_back({(int32_t)u, row_ptr[u], (int32_t)n, -1});
     } else {
       const int64_t nc = (n + split - 1) / split;
+      const int64_t base = n / nc, rem = n % nc;  // the first rem chunks take base + 1 ratings
       hv.push_back({(int32_t)u, (int32_t)slot, (int32_t)nc, 0});
-      for (int64_t c = 0; c < nc; ++c)
-        t.push_back({(int32_t)u, (int32_t)(row_ptr[u] + c * split), (int32_t)std::min<int64_t>(split, n - c * split),
-                     (int32_t)(slot + c)});
+      int64_t off = row_ptr[u];
+      for (int64_t c = 0; c < nc; ++c) {
+        const int64_t len = base + (c < rem ? 1 : 0);
+        t.push_back({(int32_t)u, (int32_t)off, (int32_t)len, (int32_t)(slot + c)});
+        off += len;
+      }
       slot += nc;
     }
   }
@@ -113,6 +119,7 @@ struct Plan {
   int64_t wg_t0[cumf::kLargeDualWgMaxBlocks + 1] = {}, wg_t1[cumf::kLargeDualWgMaxBlocks + 1] = {};
   int split = 0;
   int64_t bucket_t0[kBuckets] = {}, bucket_t1[kBuckets] = {};
+  bool bucket_partial[kBuckets] = {};  // bucket b holds a split row's chunk
   DevBuf tasks, heavy, partial;
 
   int build(const int32_t* row_ptr, int64_t m, int split_, int f) {
@@ -124,12 +131,16 @@ struct Plan {
     nnz = row_ptr[m];
     split = f <= kMaxFusedF ? split_ : 0;
     ntasks = (int64_t)t.size();
-    for (int b = 0; b < kBuckets; ++b) bucket_t0[b] = bucket_t1[b] = 0;
+    for (int b = 0; b < kBuckets; ++b) {
+      bucket_t0[b] = bucket_t1[b] = 0;
+      bucket_partial[b] = false;
+    }
     for (int64_t i = 0; i < ntasks; ++i) {
       int b = 0;
       while (b + 1 < kBuckets && t[(size_t)i].count >= kBucketEdge[b + 1]) ++b;
       if (bucket_t1[b] == 0) bucket_t0[b] = i;
       bucket_t1[b] = i + 1;
+      bucket_partial[b] = bucket_partial[b] || t[(size_t)i].slot >= 0;
     }
     light0 = ntasks;
     for (int b = 0; b < kBuckets && kBucketEdge[b + 1] <= cumf::kDualMaxRatingsHost + 1; ++b)
@@ -340,6 +351,9 @@ hipError_t dual_launch(const cumf::SweepArgs& a, int variant, hipStream_t s) {
 // ratings): 0, 1 and 2.
 constexpr int kDualBuckets = 3;
 static_assert(kBucketEdge[kDualBuckets] == cumf::kDualMaxRatingsHost + 1, "dual-form buckets hold at most 95 ratings");
+// 32-rating chunks the dual form stages for bucket b: one up to 31 ratings,
+// two up to 63, three up to 95.
+int dual_chunks_of(int b) { return kBucketEdge[b + 1] <= 32 ? 1 : kBucketEdge[b + 1] <= 64 ? 2 : 3; }
 
 // Launch of the tasks of degree buckets b_lo .. b (a contiguous range of the
 // task list: tasks are ordered by decreasing rating count) with one shape.
@@ -354,8 +368,11 @@ hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::Sw
   x.occupancy = cfg.persistent ? 0 : cfg.occupancy;
   x.task_counter = h->counters.as<int>() + side * kBuckets + b;
   if (cfg.solver == CUMF_SOLVER_DUAL) {
-    // staged 32-rating chunks: two up to 63 ratings, three up to 95
-    x.dual_chunks = kBucketEdge[b + 1] <= 64 ? 2 : 3;
+    x.dual_partial = 0;
+    for (int bb = b_lo; bb <= b; ++bb) x.dual_partial |= p.bucket_partial[bb] ? 1 : 0;
+    // (the one-chunk kernel has no partial-Gram branch: a bucket 0 with a
+    // split row's chunk takes the two-chunk one)
+    x.dual_chunks = std::max(dual_chunks_of(b), x.dual_partial ? 2 : 1);
     return dual_launch(x, cfg.variant, stream);
   }
   if (cfg.solver >= 0) x.solver = cfg.solver;
@@ -455,9 +472,9 @@ int launch_tasks(cumf_handle* h, int side, const Plan& p, const cumf::SweepArgs&
     return x.persistent == 0 && y.persistent == 0 && x.variant == y.variant && x.solver == y.solver &&
            x.occupancy == y.occupancy;
   };
-  // (the dual form stages two chunks for rows of up to 63 ratings, three
-  // above: those buckets do not share a launch)
-  auto dual_ch = [&](int b) { return kBucketEdge[b + 1] <= 64 ? 2 : 3; };
+  // (the dual form stages one, two or three chunks by bucket: buckets with
+  // different counts do not share a launch)
+  auto dual_ch = [&](int b) { return dual_chunks_of(b); };
   int grp_hi[kBuckets], grp_lo[kBuckets], ng = 0;
   for (int b = kBuckets - 1; b >= 0; --b) {
     if (p.bucket_t1[b] <= p.bucket_t0[b]) continue;

@@ -1465,8 +1465,8 @@ __device__ __forceinline__ void solve_row(f4 (&acc)[num_tiles(NB)], const SweepA
 // Cholesky costs a fraction of the primal one (the primal Gram's MFMA work
 // is about the same as the dual's).
 //   * Y is gathered as in gram_range_bf16: chunks of 32 rows, NB tiles of
-//     32 rows x 16 bf16, all CH chunks (2 for rows of up to 63 ratings, 3
-//     up to 95) at once (rows past n are zeros).
+//     32 rows x 16 bf16, all CH chunks (1 for rows of up to 31 ratings, 2 up
+//     to 63, 3 up to 95) at once (rows past n are zeros).
 //   * K = Y Y^T on v_mfma_f32_16x16x32_bf16: for k-step s and row block I,
 //     lane (kg, i) supplies row 16I + i, columns 32s + 8kg .. +7 (one
 //     16-byte LDS read; columns past the staged tiles read a zero block).
@@ -1584,6 +1584,7 @@ __device__ __forceinline__ void dual_solve_row(const SweepArgs& a, const Task& t
   bool ok;
   const int nbd = (n + 16) >> 4;  // blocks of the n x n system plus r's column
   if (nbd == 1) ok = dual_blocks<NB, CH, 1>(a, n, lds, zs, rs, zero, lane);
+  else if constexpr (CH == 1) ok = dual_blocks<NB, CH, 2>(a, n, lds, zs, rs, zero, lane);
   else if (nbd == 2) ok = dual_blocks<NB, CH, 2>(a, n, lds, zs, rs, zero, lane);
   else if (nbd == 3) ok = dual_blocks<NB, CH, 3>(a, n, lds, zs, rs, zero, lane);
   else if constexpr (CH == 2) ok = dual_blocks<NB, CH, 4>(a, n, lds, zs, rs, zero, lane);

@@ -71,9 +71,12 @@ struct SweepArgs {
   // occupancy > 0 (one-task-per-group launches): at most that many
   // wavefronts per CU, enforced through the LDS each group requests.
   int occupancy;
-  // Dual-form launches: 32-rating chunks staged per row, 2 (rows of at
-  // most 63 ratings) or 3 (at most 95).
+  // Dual-form launches: 32-rating chunks staged per row, 1 (rows of at
+  // most 31 ratings), 2 (at most 63) or 3 (at most 95); dual_partial != 0
+  // if a task of the launch is a split row's chunk (its partial Gram is
+  // formed: two or three chunks only).
   int dual_chunks;
+  int dual_partial;
 };
 
 // Longest row the dual form takes (three staged chunks of 32 ratings).

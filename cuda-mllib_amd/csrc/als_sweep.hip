@@ -179,8 +179,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
 
 #if CUMF_SWEEP_SOLVER == 0
 // Dual form of the tasks of one launch, each of at most kDualMaxRatings
-// ratings (BF16 factors).  The last chunk of a split row can be that short:
-// such a task still forms its partial Gram.
+// ratings (BF16 factors).  A split row's chunk can be that short (with a
+// small split size): such a task still forms its partial Gram.
 template <int NB, int CH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_dual_kernel(SweepArgs a) {
   extern __shared__ float lds[];
@@ -197,6 +197,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
     return;
   }
   dual_solve_row<NB, CH>(a, t, lds, lane);
+}
+// The dual form of whole rows of at most 31 ratings: one staged chunk and
+// no partial-Gram branch, so far fewer registers and less LDS per
+// wavefront (the host launches it only where no task is a split row's
+// chunk).
+template <int NB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_dual_short_kernel(SweepArgs a) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const Task t = a.tasks[blockIdx.x];
+  if (t.slot >= 0) {
+    if (lane == 0) atomicAdd(a.fail_counter, 1);  // (the host does not launch this)
+    return;
+  }
+  dual_solve_row<NB, 1>(a, t, lds, lane);
 }
 #endif
 
@@ -237,12 +252,17 @@ static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which) {
       // which == 4: the dual form (BF16 factors only)
       if (!bf) return hipErrorInvalidValue;
       if (a.ntasks > 0) {
-        if (a.dual_chunks == 2) {
-          const size_t shd = sizeof(float) * (size_t)cmax(dual_lds_floats<NB, 2>(), lds_floats_nb(NB, true));
-          hipLaunchKernelGGL((als_dual_kernel<NB, 2>), dim3((unsigned)a.ntasks), dim3(64), shd, s, a);
+        const dim3 grid((unsigned)a.ntasks), block(64);
+        const size_t gram = sizeof(float) * (size_t)lds_floats_nb(NB, true);  // the partial-Gram branch
+        if (a.dual_chunks == 1 && !a.dual_partial) {
+          const size_t shd = sizeof(float) * (size_t)dual_lds_floats<NB, 1>();
+          hipLaunchKernelGGL((als_dual_short_kernel<NB>), grid, block, shd, s, a);
+        } else if (a.dual_chunks == 2) {
+          const size_t shd = sizeof(float) * (size_t)dual_lds_floats<NB, 2>();
+          hipLaunchKernelGGL((als_dual_kernel<NB, 2>), grid, block, shd > gram ? shd : gram, s, a);
         } else if (a.dual_chunks == 3) {
-          const size_t shd = sizeof(float) * (size_t)cmax(dual_lds_floats<NB, 3>(), lds_floats_nb(NB, true));
-          hipLaunchKernelGGL((als_dual_kernel<NB, 3>), dim3((unsigned)a.ntasks), dim3(64), shd, s, a);
+          const size_t shd = sizeof(float) * (size_t)dual_lds_floats<NB, 3>();
+          hipLaunchKernelGGL((als_dual_kernel<NB, 3>), grid, block, shd > gram ? shd : gram, s, a);
         } else {
           return hipErrorInvalidValue;
         }

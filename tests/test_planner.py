@@ -1,6 +1,7 @@
 """The host task planner of the half-sweep (cumf_debug_plan; CPU only): every
 rating is covered by exactly one task, tasks run longest first, split rows
-own consecutive partial slots, and f > 140 plans whole rows."""
+own consecutive partial slots in near-equal chunks, and f > 140 plans whole
+rows."""
 import ctypes as C
 
 import numpy as np
@@ -47,11 +48,13 @@ def test_plan_covers_every_rating_once(split, f):
     if f > 140:
         assert H.shape[0] == 0 and np.all(T[:, 3] == -1)
         return
-    # split rows: consecutive slots, chunks of at most `split` ratings
+    # split rows: consecutive slots, ceil(n / split) chunks of at most
+    # `split` ratings whose sizes differ by at most one
     np.testing.assert_array_equal(H[:, 0], np.flatnonzero(deg > split))
     for row, slot0, nch, _ in H:
         mine = T[T[:, 0] == row]
-        assert mine.shape[0] == nch and np.all(mine[:, 2] <= split)
+        assert mine.shape[0] == nch == -(-deg[row] // split) and np.all(mine[:, 2] <= split)
+        assert mine[:, 2].max() - mine[:, 2].min() <= 1
         np.testing.assert_array_equal(np.sort(mine[:, 3]), np.arange(slot0, slot0 + nch))
         assert mine[:, 2].sum() == deg[row]
     if H.shape[0]:
