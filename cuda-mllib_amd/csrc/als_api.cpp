@@ -52,9 +52,8 @@ constexpr int kBuckets = 8;
 constexpr int64_t kBucketEdge[kBuckets + 1] = {0, 32, 64, 96, 256, 1024, 4096, 16384, INT64_MAX};
 constexpr int kAuxStreams = 3;
 
-// Above kMaxFusedF, rows of at most kDualMaxRatingsHost ratings (degree
-// buckets 0 .. 2) are solved in the dual form unless a test switched it off
-// (cumf_debug_large_dual).
+// Above kMaxFusedF, lighter rows are solved in the dual forms (see
+// large_launches) unless a test switched them off (cumf_debug_large_dual).
 std::atomic<int> g_large_dual{1};
 
 // Launch shape of one bucket: register budget (wavefronts per SIMD the
@@ -611,12 +610,12 @@ int run_half_sweep(cumf_handle* h, int side, const Plan& p, const DevBuf& rowp, 
       if (s0) (void)hipEventRecord(s0, h->stream);
     }
     int nlaunch = 0;
+    const size_t npend = h->timer.pending.size();
     CUMF_TRY(timed(h, Timer::TASK, [&] {
       return large_launches(a, p, &nlaunch, h->stream, h->aux[0], h->aux[1], h->fork_ev, h->join_ev[0],
                             h->join_ev[1]);
     }));
-    if (h->opt.timing && !h->capturing && !h->timer.pending.empty())
-      h->timer.pending.back().launches = std::max(1, nlaunch);
+    if (h->timer.pending.size() > npend) h->timer.pending.back().launches = std::max(1, nlaunch);
     if (s0 && s1) {
       (void)hipEventRecord(s1, h->stream);
       h->timer.pending.push_back({s0, s1, Timer::SWEEP, 1});
