@@ -369,6 +369,16 @@ hipError_t bucket_launch(cumf_handle* h, int side, const Plan& p, const cumf::Sw
   if (cfg.solver == CUMF_SOLVER_DUAL) {
     x.dual_partial = 0;
     for (int bb = b_lo; bb <= b; ++bb) x.dual_partial |= p.bucket_partial[bb] ? 1 : 0;
+    if (x.precision == CUMF_PREC_FP32) {
+      // FP32 factors: the wavefront dual form reads the rows in place (any
+      // ldf <= 256) and solves whole rows only; a launch holding a split
+      // row's chunk (a split size under 2 x 96) takes the primal kernel.
+      if (x.dual_partial) {
+        x.solver = CUMF_SOLVER_CHOLESKY;
+        return task_launch(x, cfg.variant, stream);
+      }
+      return cumf::launch_large_dual_kernel(x, x.ntasks, stream);
+    }
     // (the one-chunk kernel has no partial-Gram branch: a bucket 0 with a
     // split row's chunk takes the two-chunk one)
     x.dual_chunks = std::max(dual_chunks_of(b), x.dual_partial ? 2 : 1);
@@ -1325,7 +1335,7 @@ int autotune_side(cumf_handle* h, int side, int reps, const TuneStart& start) {
     double bk = 1e300;
     BucketCfg bc;
     std::vector<int> sols = solvers;
-    if (h->opt.precision != CUMF_PREC_FP32 && k < kDualBuckets) sols.push_back(CUMF_SOLVER_DUAL);
+    if (k < kDualBuckets) sols.push_back(CUMF_SOLVER_DUAL);
     for (int sol : sols)
       for (int variant : {2, 1})
         for (int pers : {0, h->num_cus * (variant == 1 ? 4 : 8), -7, -6, -3}) {
@@ -1443,7 +1453,7 @@ int cumf_save_tile_table(cumf_handle* h, const char* path) {
   std::fprintf(fp, "# cumf tile table: f %d precision %d solver %d\n", h->f, h->opt.precision, h->opt.solver);
   std::fprintf(fp, "# side <0 X | 1 Theta> split <ratings per task> finish <1|2> fallback <1|2>\n");
   std::fprintf(fp, "# bucket <side> <b> <min ratings> <max ratings> variant <1|2> persistent <wavefronts>"
-                  " solver <-1 handle's | 0 Cholesky | 1 CG> occupancy <wavefronts per CU, 0 any>\n");
+                  " solver <-1 handle's | 0 Cholesky | 1 CG | 2 dual> occupancy <wavefronts per CU, 0 any>\n");
   for (int side = 0; side < 2; ++side) {
     const Plan& p = side == 0 ? h->plan_x : h->plan_t;
     std::fprintf(fp, "side %d split %d finish %d fallback %d\n", side, p.split > 0 ? p.split : h->opt.split,
@@ -1485,11 +1495,11 @@ int cumf_load_tile_table(cumf_handle* h, const char* path) {
       // fallback list and kernel); Cholesky is always available.
       if (nf == 6) sol = -1;
       if (nf <= 7) occ = 0;
-      // The dual form needs BF16 factors and rows of at most 95 ratings
-      // (buckets 0 to 2), one task per group.
+      // The dual form needs rows of at most 95 ratings (buckets 0 to 2),
+      // one task per group.
       if (side < 0 || side > 1 || b < 0 || b >= kBuckets || (v != 1 && v != 2) || pers < 0 || pers > (1 << 20) ||
           sol < -1 || sol > 2 || (sol == CUMF_SOLVER_CG && h->opt.solver != CUMF_SOLVER_CG) ||
-          (sol == CUMF_SOLVER_DUAL && (h->opt.precision == CUMF_PREC_FP32 || b >= kDualBuckets || pers > 0)) ||
+          (sol == CUMF_SOLVER_DUAL && (b >= kDualBuckets || pers > 0)) ||
           occ < 0 || occ > cumf::kMaxOccupancy || (occ > 0 && occ < cumf::kMinOccupancy))
         rc = CUMF_ERR_INVALID;
       else tiles[side][b] = {v, pers, sol, occ};

@@ -86,11 +86,14 @@ void doALSWithCSR(hipStream_t cuda_stream, int* csrRowIndex, int* csrColIndex, f
                                    stays above cg_tol fall back to the Cholesky
                                    solve */
 #define CUMF_SOLVER_DUAL 2      /* tile tables only, degree buckets 0, 1 and 2 (at
-                                   most 95 ratings) of a CUMF_PREC_BF16_GRAM
-                                   handle: the same x from the n x n system
-                                   (Y Y^T + lambda n I) z = r, x = Y^T z, with
-                                   Y the row's n gathered factor rows (FP32
-                                   Cholesky of the smaller system) */
+                                   most 95 ratings): the same x from the n x n
+                                   system (Y Y^T + lambda n I) z = r, x = Y^T z,
+                                   with Y the row's n gathered factor rows (K
+                                   on BF16 MFMA from the BF16 copies, or on FP32
+                                   MFMA from the factors on a CUMF_PREC_FP32
+                                   handle; FP32 Cholesky of the smaller system).
+                                   On an FP32 handle a launch holding a split
+                                   row's chunk takes the Cholesky kernel */
 
 #define CUMF_WARM_PREV 0        /* CG starts from the row's current factor */
 #define CUMF_WARM_ZERO 1        /* CG starts from zero */
@@ -255,7 +258,8 @@ int cumf_csr_to_csc_device(hipStream_t stream, int64_t m, int64_t n, int64_t nnz
  * budget (1 or 2 wavefronts per SIMD) of the split-row finish and of the CG
  * fallback kernels, and per bucket the register budget, whether the bucket
  * runs as a persistent grid of N wavefronts taking tasks from a counter, its
- * solver (the handle's, Cholesky, or -- on a CG handle -- CG), and a cap on
+ * solver (the handle's, Cholesky, the dual form in buckets 0-2, or -- on a
+ * CG handle -- CG), and a cap on
  * the wavefronts per CU of a one-task-per-group launch (3..16; 0 = none),
  * enforced through the LDS each group requests.  Without
  * a table every half-sweep is one launch of all tasks,

@@ -130,6 +130,39 @@ def test_dual_form_matches_primal_and_oracle(tmp_path, f):
     assert res.max() <= 1e-5 and err.max() <= 1e-3, (res.max(), err.max())
 
 
+@pytest.mark.parametrize("f", [32, 100])
+def test_fp32_dual_form_matches_primal_and_oracle(tmp_path, f):
+    """On an FP32 handle the dual form of buckets 0-2 (K = Y Y^T on FP32 MFMA
+    from the factors, one wavefront per row) gives the primal solution:
+    within 1e-4 of the FP32 primal path, residual <= 1e-5 against the FP64
+    system.  With split 48 the launches that hold split rows' chunks take
+    the Cholesky kernel: those rows match the primal path bit for bit."""
+    ds = _data.netflix_like_small()
+    lam = 0.058
+    T0 = cumf.init_factors(ds.n, f, 47)
+    rp = ds.csr_row_ptr
+    deg = np.diff(rp)
+    light = deg <= 95
+    for split in (16384, 48):
+        opt = dict(solver=G.CUMF_SOLVER_CHOLESKY, split=split)
+        Xp, _, _ = _run(ds, f, lam, T0, iters=1, **opt)
+        Xd, _, st = _run(ds, f, lam, T0, iters=1, table=_dual_table(tmp_path, split), **opt)
+        assert st["failed_rows"] == 0
+        assert _rel(Xd[light], Xp[light]) <= 1e-4, _rel(Xd[light], Xp[light])
+        if split == 48:
+            np.testing.assert_array_equal(Xd[~light], Xp[~light])
+    with G.ALS(f, lam, solver=G.CUMF_SOLVER_CHOLESKY) as h:
+        h.set_dataset(ds)
+        h.load_tile_table(str(_dual_table(tmp_path, 16384)))
+        h.set_factors(theta=T0)
+        h.update_x()
+        X1, _ = h.get_factors()
+    from tests._checks import row_errors
+    rows = np.flatnonzero((deg > 0) & light)[::7][:300]
+    res, err = row_errors(rp, ds.csr_col, ds.csr_val, T0, X1, f, lam, rows)
+    assert res.max() <= 1e-5 and err.max() <= 1e-3, (res.max(), err.max())
+
+
 def test_all_persistent_buckets_and_graph_replay(tmp_path):
     ds = _data.netflix_like_small()
     f, lam = 32, 0.058
@@ -202,11 +235,13 @@ def test_bad_tile_table_rejected(tmp_path):
             p.write_text(f"bucket 0 1 0 0 variant 2 persistent 0 solver -1 occupancy {occ}\n")
             with pytest.raises(G.CumfError):
                 h.load_tile_table(str(p))
-        # the dual form: FP32 handle, a bucket that may hold 96+ ratings, a persistent grid
-        for line in ("bucket 0 1 0 0 variant 2 persistent 0 solver 2", "bucket 0 3 0 0 variant 2 persistent 0 solver 2"):
+        # the dual form: a bucket that may hold 96+ ratings, a persistent grid
+        for line in ("bucket 0 3 0 0 variant 2 persistent 0 solver 2", "bucket 0 1 0 0 variant 2 persistent 64 solver 2"):
             p.write_text(line + "\n")
             with pytest.raises(G.CumfError):
                 h.load_tile_table(str(p))
+        p.write_text("bucket 0 1 0 0 variant 2 persistent 0 solver 2\n")  # FP32 factors: the FP32 dual form
+        h.load_tile_table(str(p))
     with G.ALS(16, 0.05, precision=G.CUMF_PREC_BF16_GRAM) as h:
         for line in ("bucket 0 3 0 0 variant 2 persistent 0 solver 2", "bucket 1 0 0 0 variant 2 persistent 64 solver 2",
                      "bucket 0 0 0 0 variant 2 persistent 0 solver 3"):
