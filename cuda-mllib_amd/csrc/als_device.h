@@ -1619,4 +1619,128 @@ __device__ __forceinline__ void dual_solve_row(const SweepArgs& a, const Task& t
   if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
 }
 
+// ---------------------------------------------------------------------------
+// The dual form of whole rows without LDS staging: the MFMA operands come
+// straight from the BF16 factor rows.  For 32-column step s and row block
+// I, lane (kg, i) loads the 16 bytes of row 16I + i, columns 32s + 8kg ..
+// +7 -- the operand the staged form reads from LDS -- two steps at a time
+// (rows past n and columns past ldb read zeros).  x = Y^T z: lane l forms
+// columns 2l + 128p and the next one, each rating's row read whole by the
+// wavefront (4 bytes a lane).  LDS holds only the Cholesky's space, z, the
+// ratings and their column indices; CH bounds the row length (32 CH - 1)
+// and with it the blocks and the registers.  Whole rows only (the host
+// launches it where no task is a split row's chunk).
+// ---------------------------------------------------------------------------
+template <int CH>
+__host__ __device__ constexpr int dual_g_lds_floats() {
+  return chol_lds_floats(dual_max_blocks(CH)) + 3 * 96;
+}
+
+template <int NB, int NBD>
+__device__ __forceinline__ bool dual_blocks_g(const SweepArgs& a, int n, const int* cs, const float* rs,
+                                              float* lds, float* zs, int lane) {
+  constexpr int S = (NB + 1) / 2;  // 32-column steps over the 16 NB columns
+  const int ldb = (int)a.ldb;
+  const __bf16* theta_b = reinterpret_cast<const __bf16*>(a.theta_b);
+  const __bf16* zb = reinterpret_cast<const __bf16*>(kZeroRow);  // (512 zero bf16 >= ldb)
+  const int kg = lane >> 4, i = lane & 15;
+  const __bf16* rp[NBD];
+#pragma unroll
+  for (int I = 0; I < NBD; ++I) {
+    const int r = 16 * I + i;
+    rp[I] = r < n ? theta_b + (int64_t)cs[r] * ldb : zb;
+  }
+  f4 acc[num_tiles(NBD)];
+#pragma unroll
+  for (int t = 0; t < num_tiles(NBD); ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s0 = 0; s0 < S; s0 += 2) {
+    bf8 op[2][NBD];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int I = 0; I < NBD; ++I) {
+        const int col = 32 * (s0 + h) + 8 * kg;
+        const __bf16* p = (s0 + h < S && col < ldb) ? rp[I] + col : zb;
+        op[h][I] = *reinterpret_cast<const bf8*>(p);
+      }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if (s0 + h < S) {
+#pragma unroll
+        for (int I = 0; I < NBD; ++I)
+#pragma unroll
+          for (int J = I; J < NBD; ++J)
+            acc[tile_index(I, J, NBD)] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(op[h][I], op[h][J], acc[tile_index(I, J, NBD)], 0, 0, 0);
+      }
+  }
+  // r into column 16 NBD - 1 of the last block column (K is zero there:
+  // rows of Y past n are zero and 16 NBD - 1 >= n), mu on the diagonal.
+  const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+  for (int I = 0; I < NBD; ++I)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c == 15) acc[tile_index(I, NBD - 1, NBD)][q] = rs[16 * I + 4 * g + q];
+  add_ridge<NBD>(acc, n, a.lambda * (float)n, lane);
+  return chol_solve<NBD>(acc, n, 16 * NBD - 1, lds, zs, 16 * NBD, lane);
+}
+
+template <int NB, int CH>
+__device__ __forceinline__ void dual_solve_row_g(const SweepArgs& a, const Task& t, float* lds, int lane) {
+  constexpr int MB = dual_max_blocks(CH);
+  float* xrow = a.x + (int64_t)t.row * a.ldf;
+  __bf16* xb = reinterpret_cast<__bf16*>(a.x_b) + (int64_t)t.row * a.ldb;
+  const int n = t.count;
+  if (n <= 0 || n > KBB * CH - 1 || t.slot >= 0) {  // (the host launches only whole rows that fit)
+    for (int i = lane; i < a.ldf; i += 64) xrow[i] = 0.0f;
+    for (int i = lane; i < a.ldb; i += 64) xb[i] = (__bf16)0.0f;
+    if (n > 0 && lane == 0) atomicAdd(a.fail_counter, 1);
+    return;
+  }
+  float* zs = lds + chol_lds_floats(MB);
+  float* rs = zs + 96;
+  int* cs = reinterpret_cast<int*>(rs + 96);
+  for (int i = lane; i < 96; i += 64) {
+    const bool in = i < n;
+    rs[i] = in ? a.val[t.start + i] : 0.0f;
+    cs[i] = in ? a.col[t.start + i] : 0;
+  }
+  lds_order();
+  bool ok;
+  const int nbd = (n + 16) >> 4;  // blocks of the n x n system plus r's column
+  if (nbd == 1) ok = dual_blocks_g<NB, 1>(a, n, cs, rs, lds, zs, lane);
+  else if constexpr (MB == 2) ok = dual_blocks_g<NB, 2>(a, n, cs, rs, lds, zs, lane);
+  else if (nbd == 2) ok = dual_blocks_g<NB, 2>(a, n, cs, rs, lds, zs, lane);
+  else if (nbd == 3) ok = dual_blocks_g<NB, 3>(a, n, cs, rs, lds, zs, lane);
+  else if constexpr (MB == 4) ok = dual_blocks_g<NB, 4>(a, n, cs, rs, lds, zs, lane);
+  else if (nbd == 4) ok = dual_blocks_g<NB, 4>(a, n, cs, rs, lds, zs, lane);
+  else if (nbd == 5) ok = dual_blocks_g<NB, 5>(a, n, cs, rs, lds, zs, lane);
+  else ok = dual_blocks_g<NB, 6>(a, n, cs, rs, lds, zs, lane);
+  lds_order();
+  // x = Y^T z (chol_solve left z = 0 if it failed); the BF16 row's padding
+  // columns of Y are zero, so are x's there.
+  const __bf16* theta_b = reinterpret_cast<const __bf16*>(a.theta_b);
+  const int ldb = (int)a.ldb;
+  for (int c0 = 2 * lane; c0 < ldb; c0 += 128) {
+    float x0 = 0.0f, x1 = 0.0f;
+#pragma unroll 8
+    for (int k = 0; k < n; ++k) {
+      const uint32_t w = *reinterpret_cast<const uint32_t*>(theta_b + (int64_t)cs[k] * ldb + c0);
+      const float zk = zs[k];
+      x0 = fmaf(zk, __uint_as_float(w << 16), x0);
+      x1 = fmaf(zk, __uint_as_float(w & 0xffff0000u), x1);
+    }
+    x0 = c0 < a.f ? x0 : 0.0f;
+    x1 = c0 + 1 < a.f ? x1 : 0.0f;
+    if (c0 < (int)a.ldf) xrow[c0] = x0;
+    if (c0 + 1 < (int)a.ldf) xrow[c0 + 1] = x1;
+    xb[c0] = (__bf16)x0;
+    xb[c0 + 1] = (__bf16)x1;
+  }
+  for (int i = ldb + lane; i < (int)a.ldf; i += 64) xrow[i] = 0.0f;  // (ldf <= ldb: none)
+  if (!ok && lane == 0) atomicAdd(a.fail_counter, 1);
+}
+
 }  // namespace cumf

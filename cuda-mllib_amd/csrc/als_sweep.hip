@@ -198,20 +198,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
   }
   dual_solve_row<NB, CH>(a, t, lds, lane);
 }
-// The dual form of whole rows of at most 31 ratings: one staged chunk and
-// no partial-Gram branch, so far fewer registers and less LDS per
-// wavefront (the host launches it only where no task is a split row's
-// chunk).
-template <int NB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_dual_short_kernel(SweepArgs a) {
+// The dual form of whole rows of at most 32 CH - 1 ratings with the MFMA
+// operands read straight from the factor rows (no staging, no partial-Gram
+// branch: fewer registers and about 10 KB of LDS per wavefront).  The host
+// launches it where no task is a split row's chunk.
+template <int NB, int CH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_dual_row_kernel(SweepArgs a) {
   extern __shared__ float lds[];
-  const int lane = threadIdx.x;
-  const Task t = a.tasks[blockIdx.x];
-  if (t.slot >= 0) {
-    if (lane == 0) atomicAdd(a.fail_counter, 1);  // (the host does not launch this)
-    return;
-  }
-  dual_solve_row<NB, 1>(a, t, lds, lane);
+  dual_solve_row_g<NB, CH>(a, a.tasks[blockIdx.x], lds, threadIdx.x);
 }
 #endif
 
@@ -254,9 +248,13 @@ static hipError_t launch_nb(const SweepArgs& a, hipStream_t s, int which) {
       if (a.ntasks > 0) {
         const dim3 grid((unsigned)a.ntasks), block(64);
         const size_t gram = sizeof(float) * (size_t)lds_floats_nb(NB, true);  // the partial-Gram branch
-        if (a.dual_chunks == 1 && !a.dual_partial) {
-          const size_t shd = sizeof(float) * (size_t)dual_lds_floats<NB, 1>();
-          hipLaunchKernelGGL((als_dual_short_kernel<NB>), grid, block, shd, s, a);
+        if (!a.dual_partial && a.dual_chunks >= 1 && a.dual_chunks <= 3) {
+          if (a.dual_chunks == 1)
+            hipLaunchKernelGGL((als_dual_row_kernel<NB, 1>), grid, block, sizeof(float) * dual_g_lds_floats<1>(), s, a);
+          else if (a.dual_chunks == 2)
+            hipLaunchKernelGGL((als_dual_row_kernel<NB, 2>), grid, block, sizeof(float) * dual_g_lds_floats<2>(), s, a);
+          else
+            hipLaunchKernelGGL((als_dual_row_kernel<NB, 3>), grid, block, sizeof(float) * dual_g_lds_floats<3>(), s, a);
         } else if (a.dual_chunks == 2) {
           const size_t shd = sizeof(float) * (size_t)dual_lds_floats<NB, 2>();
           hipLaunchKernelGGL((als_dual_kernel<NB, 2>), grid, block, shd > gram ? shd : gram, s, a);
