@@ -112,8 +112,8 @@ def cpu_baseline(ds, f, lam, theta, X, target_s):
 LIMITER = {
     ("netflix_480k_17770_99m", "bf16-cg-tuned", 100):
         "X half-sweep (dual form up to 95 ratings, Cholesky above): latency of gathers and of the "
-        "per-row solves (34% of wave time waiting on memory, 33% waiting to issue, matrix pipe 29% busy); "
-        "Theta half-sweep: BF16 gathers beyond L2 (49% of wave time waiting on memory)",
+        "per-row solves (35% of wave time waiting on memory, 37% waiting to issue, matrix pipe 32% busy); "
+        "Theta half-sweep: BF16 gathers beyond L2 (48% of wave time waiting on memory)",
 }
 
 
