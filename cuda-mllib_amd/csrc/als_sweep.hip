@@ -179,8 +179,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WA
 
 #if CUMF_SWEEP_SOLVER == 0
 // Dual form of the tasks of one launch, each of at most kDualMaxRatings
-// ratings (BF16 factors).  A split row's chunk can be that short (with a
-// small split size): such a task still forms its partial Gram.
+// ratings (BF16 factors), the rows staged in LDS.  The host launches it
+// (with two or three chunks) only where a task is a split row's chunk,
+// which a small split size can make that short: such a task still forms
+// its partial Gram.
 template <int NB, int CH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CUMF_TASK_WAVES))) void als_dual_kernel(SweepArgs a) {
   extern __shared__ float lds[];
