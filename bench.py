@@ -39,6 +39,7 @@ import os
 import platform
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -242,7 +243,7 @@ def run_path(G, ds, theta0, f, lam, args, name, opts, tune, net=None, report_row
     h.set_factors(theta=theta0)
     tuning = None
     if tune == "autotune":
-        out = os.path.join(REPO, "gpurun_out", "tile_table_%s_f%d.txt" % (args.workload, f))
+        out = os.path.join(args.out_dir, "tile_table_%s_f%d.txt" % (args.workload, f))
         os.makedirs(os.path.dirname(out), exist_ok=True)
         t1 = time.perf_counter()
         h.autotune(reps=2, path=out)
@@ -264,7 +265,7 @@ def run_path(G, ds, theta0, f, lam, args, name, opts, tune, net=None, report_row
     #    tile table is tuned again on the trained factors first (a CG bucket
     #    from a converged start needs fewer rounds than from the cold one).
     if tune == "autotune" and args.retune:
-        out = os.path.join(REPO, "gpurun_out", "tile_table_%s_f%d.txt" % (args.workload, f))
+        out = os.path.join(args.out_dir, "tile_table_%s_f%d.txt" % (args.workload, f))
         t1 = time.perf_counter()
         h.autotune(reps=2, path=out)
         tuning["retune_s"] = time.perf_counter() - t1
@@ -288,7 +289,7 @@ def run_path(G, ds, theta0, f, lam, args, name, opts, tune, net=None, report_row
         he.set_mlp(net)
     he.set_dataset(ds)
     if tune == "autotune":
-        he.load_tile_table(os.path.join(REPO, "gpurun_out", "tile_table_%s_f%d.txt" % (args.workload, f)))
+        he.load_tile_table(os.path.join(args.out_dir, "tile_table_%s_f%d.txt" % (args.workload, f)))
     elif tune:
         he.load_tile_table(tune)
     he.set_factors(X=X, theta=T)
@@ -370,6 +371,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch each half-sweep eagerly")
     ap.add_argument("--no-tune", action="store_true", help="one launch per kernel, no tile table")
     ap.add_argument("--tile-table", default="", help="load degree-bucket launch shapes from this file")
+    ap.add_argument("--out-dir", default=os.path.join(tempfile.gettempdir(), "cumf_bench"),
+                    help="folder the autotuned tile table is written to")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--no-dropin", action="store_true", help="skip timing doALSWithCSR")
     ap.add_argument("--retune", action="store_true",
