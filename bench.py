@@ -393,8 +393,8 @@ def main():
                           "GBps": 2 * args.calibrate_copy / ms / 1e6}), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; the ALS sweep is single-device here "
-                                   "(sharding across GPUs is described, not built)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the ALS sweep runs on one GPU; sharding across GPUs is not implemented, "
+                                   "so only --gpus 1 is measured", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import cumf
