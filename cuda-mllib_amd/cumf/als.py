@@ -125,7 +125,7 @@ def lib():
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name, None)
-        if fn is None:  # entry point absent from this build: calling it raises
+        if fn is None:  # entry point absent from the loaded library: calling it raises
             continue
         fn.argtypes = args
         fn.restype = res

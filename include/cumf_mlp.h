@@ -1,7 +1,7 @@
 /* cumf_mlp.h -- learned warm start for the mixed-precision solver.
  *
  * No counterpart in the reference (als/src/cuda/als.cu solves every row
- * from scratch); this is the build's learned part.  A small MLP maps the
+ * from scratch); this is the library's learned part.  A small MLP maps the
  * features of row u of an X half-sweep,
  *     [log n_u, diag(A_u) / n_u, b_u / n_u]          (2f + 1 values)
  * to a prediction of x_u, used as the starting point of the CG solve

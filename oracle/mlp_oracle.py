@@ -19,7 +19,7 @@ outputs of (Y - x*)^2.  Backward (the rounding points the kernels use):
     dZ1 = bf16((dZ2 W2) * gelu'(Z1)), dW1 = dZ1^T H0
 Adam (Kingma & Ba 2015, Algorithm 1) on FP32 master weights.
 
-This is the build's own network (the reference has no learned part); the
+This is the library's own network (the reference has no learned part); the
 functions are pinned by central finite differences of the unrounded FP64
 loss in tests/test_mlp_oracle.py.
 """

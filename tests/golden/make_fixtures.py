@@ -8,7 +8,7 @@ FP64 checker under oracle/, which these fixtures pin):
   b_u = Y_u^T r_u                                als/src/cuda/als.cu:1039-1054
   x_u = A_u^{-1} b_u (x_u = 0 when n_u = 0)       als/src/cuda/als.cu:935-997
   compat: A(10I+4, 10J) := G(10I, 10J), I <= J   als/src/cuda/als.cu:275, 718
-and of this build's iteration/RMSE definitions (X sweep, then Theta sweep;
+and of this library's iteration/RMSE definitions (X sweep, then Theta sweep;
 RMSE over a held-out COO set, no clipping).
 
 Run:  python tests/golden/make_fixtures.py   (numpy + scipy, seconds)
