@@ -1,7 +1,7 @@
 /*
  * als_oracle.c -- FP64 CPU restatement of the cuMF ALS normal-equation update.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: the parity tests, the
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: the parity tests, the
  * smoke check and the benchmark's CPU-baseline leg call it; the product
  * library (libGPUALS.so) never links, loads or calls it.
  *
@@ -22,7 +22,7 @@
  *
  * The reference has no CPU ALS, no sweep loop and no RMSE (those live in
  * Spark).  The sweep order, held-out RMSE and objective below are this
- * checker's definitions: one iteration = X half-sweep from Theta over
+ * oracle's definitions: one iteration = X half-sweep from Theta over
  * CSR(R), then Theta half-sweep from X over CSR(R^T).
  *
  * Everything is computed in FP64.  Rows with n_u = 0 give x_u = 0 (the

@@ -1,6 +1,6 @@
 """FP64 numpy restatement of the warm-start MLP (forward, backward, Adam).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the HIP
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the oracle of the HIP
 MLP kernels, never by the product path.
 
 The network maps the per-row features of one half-sweep,

@@ -1,7 +1,7 @@
-"""ctypes binding of the FP64 CPU checker, oracle/libals_oracle.so.
+"""ctypes binding of the FP64 CPU oracle, oracle/libals_oracle.so.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-the CPU-baseline leg of bench.py, always as the checker or the timed CPU
+the CPU-baseline leg of bench.py, always as the oracle or the timed CPU
 baseline, never as part of the product path.  See als_oracle.c for the
 reference lines each function restates.
 """

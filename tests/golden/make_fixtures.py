@@ -3,7 +3,7 @@
 The reference ships no tests, golden vectors or fixtures for this path, so
 the expected values here come from an independent dense numpy/scipy
 computation of the definitions the reference implements (not from the
-FP64 checker under oracle/, which these fixtures pin):
+FP64 oracle under oracle/, which these fixtures pin):
   A_u = Y_u^T Y_u + fl32(lambda * n_u) I        als/src/cuda/als.cu:47-51, 905-909
   b_u = Y_u^T r_u                                als/src/cuda/als.cu:1039-1054
   x_u = A_u^{-1} b_u (x_u = 0 when n_u = 0)       als/src/cuda/als.cu:935-997

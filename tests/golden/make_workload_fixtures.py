@@ -9,7 +9,7 @@ regenerated the same ratings), the held-out RMSE after each iteration, and
 the factor rows of a fixed sample of users and items after iterations 2 and
 10 (FP64, stored as float64).
 
-The expected values come from the FP64 CPU checker (oracle/), whose
+The expected values come from the FP64 CPU oracle (oracle/), whose
 functions the tiny fixtures of make_fixtures.py and the closed-form tests
 of tests/test_oracle.py pin; this file extends it to the full-size workload
 that a CPU run can still afford (about ten minutes on eight cores).

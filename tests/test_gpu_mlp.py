@@ -1,13 +1,13 @@
 """GPU checks of the learned warm start (include/cumf_mlp.h) against the
-FP64 numpy checker (oracle/mlp_oracle.py), through the C ABI.
+FP64 numpy oracle (oracle/mlp_oracle.py), through the C ABI.
 
 Bars:
-  * initial weights equal the checker's generator bit for bit;
-  * forward: ||Y_gpu - Y_64|| <= 5e-3 ||Y_64|| (the checker rounds to BF16
+  * initial weights equal the oracle's generator bit for bit;
+  * forward: ||Y_gpu - Y_64|| <= 5e-3 ||Y_64|| (the oracle rounds to BF16
     where the kernels do; the residue is FP32 vs FP64 accumulation moving an
     occasional BF16 rounding);
   * gradients: per tensor ||g_gpu - g_64|| <= 2e-2 ||g_64||, loss to 1e-4;
-  * Adam: one step on the GPU's own gradient matches the checker's update
+  * Adam: one step on the GPU's own gradient matches the oracle's update
     to 1e-6 absolute;
   * features: within 2^-7 relative of the FP64 features (BF16 storage);
   * training on harvested sweeps lowers the loss; the network's start
@@ -38,7 +38,7 @@ def _rand_batch(net, B, seed):
 
 
 @pytest.mark.parametrize("f", [32, 100])
-def test_init_and_forward_match_checker(f):
+def test_init_and_forward_match_oracle(f):
     with NN.MLP(f, seed=7) as net:
         P = net.get_params()
         ref = M.init_params(f, 7)
@@ -52,7 +52,7 @@ def test_init_and_forward_match_checker(f):
 
 
 @pytest.mark.parametrize("f", [32, 100])
-def test_gradients_match_checker(f):
+def test_gradients_match_oracle(f):
     with NN.MLP(f, seed=3) as net:
         # non-zero biases so their gradients matter
         P = net.get_params()
@@ -73,7 +73,7 @@ def test_gradients_match_checker(f):
         assert err <= 2e-2, (k, err)
 
 
-def test_adam_step_matches_checker():
+def test_adam_step_matches_oracle():
     f = 32
     with NN.MLP(f, seed=4) as net:
         P0 = net.get_params()
@@ -89,7 +89,7 @@ def test_adam_step_matches_checker():
         assert np.abs(a - b).max() <= 1e-6
 
 
-def test_features_match_checker():
+def test_features_match_oracle():
     ds = _data.netflix_like_small()
     f, lam = 32, 0.058
     T0 = cumf.init_factors(ds.n, f, 9)

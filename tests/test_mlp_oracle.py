@@ -1,4 +1,4 @@
-"""Pins the FP64 MLP checker (oracle/mlp_oracle.py) on the CPU.
+"""Pins the FP64 MLP oracle (oracle/mlp_oracle.py) on the CPU.
 
   * backward(): central finite differences of the unrounded FP64 loss on a
     small network (f=4: 9 -> 256 -> 256 -> 4 real units) agree with the

@@ -1,5 +1,5 @@
-"""Pins the FP64 CPU checker (oracle/) before it is trusted as the parity
-oracle.  Each check is independent of the checker's own code: committed
+"""Pins the FP64 CPU oracle (oracle/) before it is trusted as the parity
+oracle.  Each check is independent of the oracle's own code: committed
 golden vectors from a dense numpy/scipy computation, scipy solvers,
 closed forms, invariants and special cases, chosen so that a dropped term
 (ridge, RHS), a wrong index (transposed Gram, wrong factor row) or a wrong
